@@ -1,0 +1,872 @@
+// Host side of the MI355X (T) path: tiling and task planning (the fused
+// driver's task loop), the seeded synthetic generator, HBM residency of the
+// inputs, the persistent launch and the C ABI of include/exachem_t.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/exachem_t.h"
+#include "xt_device.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define XT_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if(e_ != hipSuccess)                                                                \
+      return fail(XT_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while(0)
+
+// ------------------------------------------------------------- generator
+inline uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+inline uint64_t hash3(uint64_t seed, uint64_t stream, uint64_t idx) {
+  return mix64(mix64(seed ^ (stream * 0xD6E8FEB86659FD93ULL)) ^ mix64(idx));
+}
+inline double u01(uint64_t h) { return ((double) (h >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+inline double gauss(uint64_t seed, uint64_t stream, uint64_t idx) {
+  const double u1 = u01(hash3(seed, stream, 2 * idx));
+  const double u2 = u01(hash3(seed, stream, 2 * idx + 1));
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586476925286766559 * u2);
+}
+
+template<class F>
+void parallel_for(long n, F f) {
+  unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  if(n < 64) nt = 1;
+  std::vector<std::thread> th;
+  std::atomic<long>        next{0};
+  auto                     body = [&]() {
+    for(;;) {
+      long i = next.fetch_add(1);
+      if(i >= n) break;
+      f(i);
+    }
+  };
+  for(unsigned t = 1; t < nt; t++) th.emplace_back(body);
+  body();
+  for(auto& t: th) t.join();
+}
+
+enum Stream : uint64_t { S_EO = 1, S_EV = 2, S_VVOO = 3, S_OVOO = 4, S_VVVO = 5, S_T1 = 6 };
+
+int synth(int O, int V, uint64_t seed, double sigma, double t1s, double* eo, double* ev,
+          double* t1, double* t2, double* vvoo, double* ovoo, double* vvvo) {
+  if(O <= 0 || V <= 0) return fail(XT_EINVAL, "synth: O and V must be positive");
+  std::vector<double> eo_(O), ev_(V);
+  for(int i = 0; i < O; i++) eo_[i] = -2.0 + 1.7 * u01(hash3(seed, S_EO, i));
+  for(int a = 0; a < V; a++) ev_[a] = 0.1 + 2.9 * u01(hash3(seed, S_EV, a));
+  std::sort(eo_.begin(), eo_.end());
+  std::sort(ev_.begin(), ev_.end());
+  if(eo) std::copy(eo_.begin(), eo_.end(), eo);
+  if(ev) std::copy(ev_.begin(), ev_.end(), ev);
+  const long o = O, v = V;
+  if(t1)
+    for(long a = 0; a < v; a++)
+      for(long i = 0; i < o; i++) t1[a * o + i] = t1s * gauss(seed, S_T1, a * o + i);
+  if(vvoo || t2) {
+    // (ai|bj) = (bj|ai): value drawn at the lexicographically smaller of (a,i,b,j), (b,j,a,i)
+    parallel_for(v, [&](long a) {
+      for(long b = 0; b < v; b++)
+        for(long i = 0; i < o; i++)
+          for(long j = 0; j < o; j++) {
+            bool   sw = (b < a) || (b == a && j < i);
+            long   a_ = sw ? b : a, i_ = sw ? j : i, b_ = sw ? a : b, j_ = sw ? i : j;
+            double x  = sigma * gauss(seed, S_VVOO, ((a_ * o + i_) * v + b_) * o + j_);
+            size_t id = ((a * v + b) * o + i) * o + j;
+            if(vvoo) vvoo[id] = x;
+            if(t2) t2[id] = x / ((eo_[i] + eo_[j]) - (ev_[a] + ev_[b]));
+          }
+    });
+  }
+  if(ovoo) {
+    parallel_for(o, [&](long i) {
+      for(long a = 0; a < v; a++)
+        for(long j = 0; j < o; j++)
+          for(long k = 0; k < o; k++) {
+            long jm = std::min(j, k), kM = std::max(j, k);
+            ovoo[((i * v + a) * o + j) * o + k] =
+              sigma * gauss(seed, S_OVOO, ((i * v + a) * o + jm) * o + kM);
+          }
+    });
+  }
+  if(vvvo) {
+    parallel_for(v, [&](long a) {
+      for(long b = 0; b < v; b++) {
+        long am = std::min(a, b), bM = std::max(a, b);
+        for(long c = 0; c < v; c++)
+          for(long i = 0; i < o; i++)
+            vvvo[((a * v + b) * v + c) * o + i] =
+              sigma * gauss(seed, S_VVVO, ((am * v + bM) * v + c) * o + i);
+      }
+    });
+  }
+  return XT_OK;
+}
+
+// ------------------------------------------------------------- planning
+constexpr int kSubsPerUnit = 32;
+
+inline void pair_of(int s, int& a, int& b) {
+  a = (s == 0) ? 1 : 0;
+  b = (s == 2) ? 1 : 2;
+}
+inline int sgn_single(int x) { return x == 1 ? -1 : 1; }
+
+struct Plan {
+  int                     O = 0, V = 0;
+  bool                    restricted = true;
+  std::vector<int>        k_range, k_spin, spatial_off;
+  int                     noab = 0, nvab = 0;
+  std::vector<long>       tasks;  // 6 per task, tile ids
+  std::vector<double>     factors;
+  std::vector<xt::TaskDev> dev;
+  long                    n_units = 0;
+};
+
+// Slot logic of one task, derived from the tile spins.  It restates the
+// enable and operand rules of the gathers (singles.hpp:100-119,
+// doubles1.hpp:94-117, doubles2.hpp:98-121): a d1 equation with single hole
+// hs and particle pair (pA,pB) runs over h7 of spin s(pA)+s(pB)-s(hs), a d2
+// equation with hole pair (hA,hB) and single particle ps over p7 of spin
+// s(hA)+s(hB)-s(ps); s1 needs s(pa) = s(hi).  The other enable tests of the
+// gathers (tile order, spin total, restricted total != 12) always hold for
+// tasks that pass the driver filter, since every tuple is a permutation of
+// the task.
+void slot_codes(const int* sp, uint64_t& d1, uint64_t& d2, uint64_t& s1, int& nd1, int& nd2,
+                int& ns1) {
+  d1 = d2 = s1 = 0;
+  nd1 = nd2 = ns1 = 0;
+  for(int hs = 0; hs < 3; hs++)
+    for(int ps = 0; ps < 3; ps++) {
+      const int g = 3 * hs + ps;
+      int       hA, hB, pA, pB;
+      pair_of(hs, hA, hB);
+      pair_of(ps, pA, pB);
+      const int sHS = sp[hs], sHA = sp[hA], sHB = sp[hB];
+      const int sPS = sp[3 + ps], sPA = sp[3 + pA], sPB = sp[3 + pB];
+      const bool neg = (-sgn_single(hs) * sgn_single(ps)) < 0;
+      {
+        const int s7 = sPA + sPB - sHS;
+        if(s7 == 1 || s7 == 2) {
+          uint64_t c = 1 | (neg ? 2 : 0);
+          if(sPA == sHS && sPB == s7) c |= 4;   // t(pA,pB,hs,h7) term
+          if(sPA == s7 && sPB == sHS) c |= 8;   // -t(pA,pB,h7,hs) term
+          if(sHA == s7 && sHB == sPS) c |= 16;  // (hA h7|hB ps)
+          if(sHB == s7 && sHA == sPS) c |= 32;  // -(hB h7|hA ps)
+          d1 |= c << (6 * g);
+          nd1++;
+        }
+      }
+      {
+        const int s7 = sHA + sHB - sPS;
+        if(s7 == 1 || s7 == 2) {
+          uint64_t c = 1 | (neg ? 2 : 0);
+          if(sHS == sPA && s7 == sPB) c |= 4;   // (hs pA|p7 pB)
+          if(sHS == sPB && s7 == sPA) c |= 8;   // -(hs pB|p7 pA)
+          if(sPS == sHA && s7 == sHB) c |= 16;  // t(ps,p7,hA,hB)
+          if(sPS == sHB && s7 == sHA) c |= 32;  // -t(p7,ps,hA,hB)
+          d2 |= c << (6 * g);
+          nd2++;
+        }
+      }
+    }
+  for(int pa = 0; pa < 3; pa++)
+    for(int hi = 0; hi < 3; hi++) {
+      const int slot = 3 * pa + hi;
+      if(sp[3 + pa] != sp[hi]) continue;
+      int hj, hk, pb, pc;
+      pair_of(hi, hj, hk);
+      pair_of(pa, pb, pc);
+      uint64_t c = 1 | ((sgn_single(hi) * sgn_single(pa)) < 0 ? 2 : 0);
+      if(sp[hk] == sp[3 + pc] && sp[hj] == sp[3 + pb]) c |= 4;  // (hk pc|hj pb)
+      if(sp[hk] == sp[3 + pb] && sp[hj] == sp[3 + pc]) c |= 8;  // -(hk pb|hj pc)
+      s1 |= c << (4 * slot);
+      ns1++;
+    }
+}
+
+int build_plan(Plan& P, int O, int V, const std::vector<int>& occ, const std::vector<int>& vir,
+               bool restricted) {
+  P = Plan();
+  P.O = O;
+  P.V = V;
+  P.restricted = restricted;
+  // |occ_a|occ_b|virt_a|virt_b| (cholesky_2e.cpp:192-277), k_spin (ccsd_t.cpp:245-249)
+  int off = 0;
+  for(int half = 0; half < 2; half++)
+    for(int t: occ) {
+      P.k_range.push_back(t);
+      P.k_spin.push_back(half + 1);
+      P.spatial_off.push_back(off % O);
+      off += t;
+    }
+  off = 0;
+  for(int half = 0; half < 2; half++)
+    for(int t: vir) {
+      P.k_range.push_back(t);
+      P.k_spin.push_back(half + 1);
+      P.spatial_off.push_back(off % V);
+      off += t;
+    }
+  P.noab = 2 * (int) occ.size();
+  P.nvab = 2 * (int) vir.size();
+  const int noab = P.noab, nvab = P.nvab;
+  const auto& ks = P.k_spin;
+  // task loop of the fused driver, seq-h3b variant (ccsd_t_fused_driver.hpp:368-395)
+  for(int h1 = 0; h1 < noab; h1++)
+    for(int p4 = noab; p4 < noab + nvab; p4++)
+      for(int h2 = h1; h2 < noab; h2++)
+        for(int p5 = p4; p5 < noab + nvab; p5++)
+          for(int p6 = p5; p6 < noab + nvab; p6++)
+            for(int h3 = h2; h3 < noab; h3++) {
+              if(ks[p4] + ks[p5] + ks[p6] != ks[h1] + ks[h2] + ks[h3]) continue;
+              if(restricted && ks[p4] + ks[p5] + ks[p6] + ks[h1] + ks[h2] + ks[h3] > 8) continue;
+              double f = restricted ? 2.0 : 1.0;
+              if(p4 == p5 && p5 == p6) f /= 6.0;
+              else if(p4 == p5 || p5 == p6) f /= 2.0;
+              if(h1 == h2 && h2 == h3) f /= 6.0;
+              else if(h1 == h2 || h2 == h3) f /= 2.0;
+              const long t[6] = {h1, h2, h3, p4, p5, p6};
+              P.tasks.insert(P.tasks.end(), t, t + 6);
+              P.factors.push_back(f);
+            }
+  const long nt = (long) P.factors.size();
+  P.dev.resize(nt);
+  long unit = 0;
+  for(long k = 0; k < nt; k++) {
+    xt::TaskDev& T = P.dev[k];
+    std::memset(&T, 0, sizeof(T));
+    int sp[6];
+    T.n_sub = 1;
+    for(int p = 0; p < 6; p++) {
+      const long b = P.tasks[6 * k + p];
+      T.dim[p]     = P.k_range[b];
+      T.off[p]     = P.spatial_off[b];
+      T.nsub[p]    = (T.dim[p] + 3) / 4;
+      T.n_sub *= T.nsub[p];
+      sp[p] = P.k_spin[b];
+    }
+    int nd1, nd2, ns1;
+    slot_codes(sp, T.d1code, T.d2code, T.s1code, nd1, nd2, ns1);
+    T.factor        = P.factors[k];
+    T.subs_per_unit = kSubsPerUnit;
+    T.n_units       = (T.n_sub + kSubsPerUnit - 1) / kSubsPerUnit;
+    T.unit_begin    = (int) unit;
+    unit += T.n_units;
+    double prod = 1;
+    for(int p = 0; p < 6; p++) prod *= T.dim[p];
+    // helper_calculate_num_ops (fused_common.hpp:131-261): 2*prod per s1 slot,
+    // 2*prod*|h7 range| per d1 slot, 2*prod*|p7 range| per d2 slot; closed-shell
+    // spin halves have O (V) orbitals each.
+    T.flops = 2.0 * prod * (ns1 + (double) nd1 * O + (double) nd2 * V);
+  }
+  if(unit > (long) 0x7fffffff) return fail(XT_EINVAL, "too many work units");
+  P.n_units = unit;
+  return XT_OK;
+}
+
+int current_device_is_gfx950(int dev) {
+  hipDeviceProp_t prop;
+  if(hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+}  // namespace
+
+// =============================================================== context
+struct xt_ctx {
+  int         device = 0;
+  hipStream_t stream = nullptr;
+  int         n_cu   = 0;
+  Plan        plan;
+  bool        have_plan = false, have_inputs = false;
+  // device inputs
+  double *eo = nullptr, *ev = nullptr, *t1 = nullptr, *t2h = nullptr, *t2p = nullptr,
+         *ovk = nullptr, *xv = nullptr, *vvoo = nullptr;
+  // device plan
+  xt::TaskDev* d_tasks = nullptr;
+  int *d_unit_task = nullptr, *d_order = nullptr;
+  double *d_unit_e = nullptr, *d_task_e = nullptr, *d_e45 = nullptr;
+  unsigned int*       d_queue   = nullptr;
+  unsigned long long *d_t0 = nullptr, *d_t1 = nullptr;
+  long                dev_tasks_n = -1;
+  hipEvent_t          ev0 = nullptr, ev1 = nullptr;
+  double              last_ms = 0.0;
+
+  void free_inputs() {
+    for(double** p: {&eo, &ev, &t1, &t2h, &t2p, &ovk, &xv, &vvoo})
+      if(*p) {
+        (void) hipFree(*p);
+        *p = nullptr;
+      }
+    have_inputs = false;
+  }
+  void free_plan_dev() {
+    if(d_tasks) (void) hipFree(d_tasks);
+    if(d_unit_task) (void) hipFree(d_unit_task);
+    if(d_order) (void) hipFree(d_order);
+    if(d_unit_e) (void) hipFree(d_unit_e);
+    if(d_task_e) (void) hipFree(d_task_e);
+    if(d_e45) (void) hipFree(d_e45);
+    if(d_queue) (void) hipFree(d_queue);
+    if(d_t0) (void) hipFree(d_t0);
+    if(d_t1) (void) hipFree(d_t1);
+    d_tasks = nullptr;
+    d_unit_task = d_order = nullptr;
+    d_unit_e = d_task_e = d_e45 = nullptr;
+    d_queue = nullptr;
+    d_t0 = d_t1 = nullptr;
+    dev_tasks_n = -1;
+  }
+};
+
+extern "C" {
+
+const char* xt_last_error(void) { return g_err.c_str(); }
+const char* xt_version(void) { return "exachem_amd-xt 0.1 (gfx950)"; }
+
+int xt_device_available(void) {
+  int n = 0;
+  if(hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+  return current_device_is_gfx950(0);
+}
+
+int xt_create(int device, xt_ctx** out) {
+  if(!out) return fail(XT_EINVAL, "xt_create: out is null");
+  *out = nullptr;
+  if(device == -1) {  // host-only context: planning (tiling, tasks, flops), no HIP calls
+    auto* c   = new xt_ctx();
+    c->device = -1;
+    *out      = c;
+    return XT_OK;
+  }
+  int n = 0;
+  if(hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(XT_ENODEV, "xt_create: no HIP device");
+  if(device < 0 || device >= n) return fail(XT_EINVAL, "xt_create: bad device index");
+  if(!current_device_is_gfx950(device))
+    return fail(XT_ENODEV, "xt_create: device is not gfx950");
+  auto* c   = new xt_ctx();
+  c->device = device;
+  XT_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  XT_HIP(hipGetDeviceProperties(&prop, device));
+  c->n_cu = prop.multiProcessorCount;
+  XT_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  XT_HIP(hipEventCreate(&c->ev0));
+  XT_HIP(hipEventCreate(&c->ev1));
+  *out = c;
+  return XT_OK;
+}
+
+int xt_destroy(xt_ctx* c) {
+  if(!c) return XT_OK;
+  if(c->device < 0) {
+    delete c;
+    return XT_OK;
+  }
+  (void) hipSetDevice(c->device);
+  (void) hipStreamSynchronize(c->stream);
+  c->free_inputs();
+  c->free_plan_dev();
+  if(c->ev0) (void) hipEventDestroy(c->ev0);
+  if(c->ev1) (void) hipEventDestroy(c->ev1);
+  if(c->stream) (void) hipStreamDestroy(c->stream);
+  delete c;
+  return XT_OK;
+}
+
+int xt_set_tiling(xt_ctx* c, int O, int V, int n_occ_tiles, const int* occ_tiles,
+                  int n_vir_tiles, const int* vir_tiles, int is_restricted) {
+  if(!c || !occ_tiles || !vir_tiles || n_occ_tiles <= 0 || n_vir_tiles <= 0)
+    return fail(XT_EINVAL, "xt_set_tiling: bad arguments");
+  std::vector<int> occ(occ_tiles, occ_tiles + n_occ_tiles), vir(vir_tiles, vir_tiles + n_vir_tiles);
+  long so = 0, sv = 0;
+  for(int t: occ) {
+    if(t <= 0) return fail(XT_EINVAL, "xt_set_tiling: empty occupied tile");
+    so += t;
+  }
+  for(int t: vir) {
+    if(t <= 0) return fail(XT_EINVAL, "xt_set_tiling: empty virtual tile");
+    sv += t;
+  }
+  if(so != O || sv != V) return fail(XT_EINVAL, "xt_set_tiling: tiles do not sum to O / V");
+  if(c->have_inputs && (c->plan.O != O || c->plan.V != V)) c->free_inputs();
+  int rc = build_plan(c->plan, O, V, occ, vir, is_restricted != 0);
+  if(rc) return rc;
+  c->have_plan = true;
+  c->free_plan_dev();
+  return XT_OK;
+}
+
+int xt_num_tasks(xt_ctx* c, long* n) {
+  if(!c || !n) return fail(XT_EINVAL, "xt_num_tasks: null argument");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_num_tasks: no tiling");
+  *n = (long) c->plan.factors.size();
+  return XT_OK;
+}
+
+int xt_task_list(xt_ctx* c, long* tasks, double* factors) {
+  if(!c) return fail(XT_EINVAL, "xt_task_list: null context");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_task_list: no tiling");
+  if(tasks) std::copy(c->plan.tasks.begin(), c->plan.tasks.end(), tasks);
+  if(factors) std::copy(c->plan.factors.begin(), c->plan.factors.end(), factors);
+  return XT_OK;
+}
+
+int xt_flops(xt_ctx* c, double* per_task, double* total, double* parts) {
+  if(!c) return fail(XT_EINVAL, "xt_flops: null context");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_flops: no tiling");
+  long double tot = 0, s1 = 0, d1 = 0, d2 = 0;
+  for(size_t k = 0; k < c->plan.dev.size(); k++) {
+    const auto& T = c->plan.dev[k];
+    if(per_task) per_task[k] = T.flops;
+    tot += T.flops;
+    long double prod = 1;
+    for(int p = 0; p < 6; p++) prod *= T.dim[p];
+    int nd1 = 0, nd2 = 0, ns1 = 0;
+    for(int g = 0; g < 9; g++) {
+      nd1 += (T.d1code >> (6 * g)) & 1;
+      nd2 += (T.d2code >> (6 * g)) & 1;
+      ns1 += (T.s1code >> (4 * g)) & 1;
+    }
+    s1 += 2 * prod * ns1;
+    d1 += 2 * prod * nd1 * c->plan.O;
+    d2 += 2 * prod * nd2 * c->plan.V;
+  }
+  if(total) *total = (double) tot;
+  if(parts) {
+    parts[0] = (double) s1;
+    parts[1] = (double) d1;
+    parts[2] = (double) d2;
+  }
+  return XT_OK;
+}
+
+int xt_synth_closed_shell(int O, int V, uint64_t seed, double sigma, double t1_sigma,
+                          double* eps_occ, double* eps_vir, double* t1, double* t2, double* vvoo,
+                          double* ovoo, double* vvvo) {
+  return synth(O, V, seed, sigma, t1_sigma, eps_occ, eps_vir, t1, t2, vvoo, ovoo, vvvo);
+}
+
+static int upload_common(xt_ctx* c, const double* eo, const double* ev, const double* t1,
+                         const double* t2, const double* vvoo, const double* ovoo,
+                         const double* vvvo) {
+  const size_t o = c->plan.O, v = c->plan.V;
+  XT_HIP(hipSetDevice(c->device));
+  c->free_inputs();
+  auto alloc = [&](double** p, size_t n) -> int {
+    hipError_t e = hipMalloc((void**) p, n * sizeof(double));
+    if(e != hipSuccess) {
+      *p = nullptr;
+      return fail(XT_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(double)) + " bytes failed");
+    }
+    return XT_OK;
+  };
+  int rc;
+  if((rc = alloc(&c->eo, o)) || (rc = alloc(&c->ev, v)) || (rc = alloc(&c->t1, v * o)) ||
+     (rc = alloc(&c->t2h, v * v * o * o)) || (rc = alloc(&c->t2p, v * v * o * o)) ||
+     (rc = alloc(&c->ovk, o * v * o * o)) || (rc = alloc(&c->xv, v * v * v * o)) ||
+     (rc = alloc(&c->vvoo, v * v * o * o))) {
+    c->free_inputs();
+    return rc;
+  }
+  hipStream_t st = c->stream;
+  XT_HIP(hipMemcpyAsync(c->eo, eo, o * sizeof(double), hipMemcpyHostToDevice, st));
+  XT_HIP(hipMemcpyAsync(c->ev, ev, v * sizeof(double), hipMemcpyHostToDevice, st));
+  XT_HIP(hipMemcpyAsync(c->t1, t1, v * o * sizeof(double), hipMemcpyHostToDevice, st));
+  XT_HIP(hipMemcpyAsync(c->vvoo, vvoo, v * v * o * o * sizeof(double), hipMemcpyHostToDevice, st));
+  // staging copies of t2 / ovoo / vvvo: t2 lands in t2h directly; ovoo in ovk;
+  // vvvo needs a scratch buffer for the relayout.
+  double* vvvo_d = nullptr;
+  if((rc = alloc(&vvvo_d, v * v * v * o))) return rc;
+  XT_HIP(hipMemcpyAsync(c->t2h, t2, v * v * o * o * sizeof(double), hipMemcpyHostToDevice, st));
+  XT_HIP(hipMemcpyAsync(c->ovk, ovoo, o * v * o * o * sizeof(double), hipMemcpyHostToDevice, st));
+  XT_HIP(hipMemcpyAsync(vvvo_d, vvvo, v * v * v * o * sizeof(double), hipMemcpyHostToDevice, st));
+  int err = xt::launch_relayout(c->t2h, c->ovk, vvvo_d, (int) o, (int) v, c->t2h, c->t2p, c->ovk,
+                                c->xv, st);
+  hipError_t se = hipStreamSynchronize(st);
+  (void) hipFree(vvvo_d);
+  if(err) return fail(XT_EHIP, std::string("relayout: ") + hipGetErrorString((hipError_t) err));
+  if(se != hipSuccess) return fail(XT_EHIP, std::string("upload: ") + hipGetErrorString(se));
+  c->have_inputs = true;
+  return XT_OK;
+}
+
+int xt_upload_closed_shell(xt_ctx* c, const double* eps_occ, const double* eps_vir,
+                           const double* t1, const double* t2, const double* vvoo,
+                           const double* ovoo, const double* vvvo) {
+  if(!c || !eps_occ || !eps_vir || !t1 || !t2 || !vvoo || !ovoo || !vvvo)
+    return fail(XT_EINVAL, "xt_upload_closed_shell: null argument");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_upload_closed_shell: call xt_set_tiling first");
+  if(c->device < 0) return fail(XT_ENODEV, "xt_upload_closed_shell: host-only context");
+  return upload_common(c, eps_occ, eps_vir, t1, t2, vvoo, ovoo, vvvo);
+}
+
+int xt_synth_upload(xt_ctx* c, uint64_t seed, double sigma, double t1_sigma) {
+  if(!c) return fail(XT_EINVAL, "xt_synth_upload: null context");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_synth_upload: call xt_set_tiling first");
+  if(c->device < 0) return fail(XT_ENODEV, "xt_synth_upload: host-only context");
+  const size_t o = c->plan.O, v = c->plan.V;
+  std::vector<double> eo(o), ev(v), t1(v * o);
+  std::vector<double> t2(v * v * o * o), vvoo(v * v * o * o), ovoo(o * v * o * o), vvvo(v * v * v * o);
+  int rc = synth((int) o, (int) v, seed, sigma, t1_sigma, eo.data(), ev.data(), t1.data(),
+                 t2.data(), vvoo.data(), ovoo.data(), vvvo.data());
+  if(rc) return rc;
+  return upload_common(c, eo.data(), ev.data(), t1.data(), t2.data(), vvoo.data(), ovoo.data(),
+                       vvvo.data());
+}
+
+void xt_default_opts(xt_opts* o) {
+  if(!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->precision        = XT_PREC_FP64;
+  o->schedule         = XT_SCHED_DRIVER;
+  o->use_graph        = 0;
+  o->record_task_time = 0;
+  o->workgroups       = 0;
+}
+
+static int ensure_plan_dev(xt_ctx* c) {
+  const Plan& P = c->plan;
+  const long  nt = (long) P.dev.size();
+  if(c->dev_tasks_n == nt) return XT_OK;
+  c->free_plan_dev();
+  std::vector<int> unit_task(std::max<long>(P.n_units, 1));
+  for(long k = 0; k < nt; k++)
+    for(int u = 0; u < P.dev[k].n_units; u++) unit_task[P.dev[k].unit_begin + u] = (int) k;
+  XT_HIP(hipMalloc((void**) &c->d_tasks, sizeof(xt::TaskDev) * std::max<long>(nt, 1)));
+  XT_HIP(hipMalloc((void**) &c->d_unit_task, sizeof(int) * unit_task.size()));
+  XT_HIP(hipMalloc((void**) &c->d_order, sizeof(int) * unit_task.size()));
+  XT_HIP(hipMalloc((void**) &c->d_unit_e, sizeof(double) * 2 * unit_task.size()));
+  XT_HIP(hipMalloc((void**) &c->d_task_e, sizeof(double) * 2 * std::max<long>(nt, 1)));
+  XT_HIP(hipMalloc((void**) &c->d_e45, sizeof(double) * 2));
+  XT_HIP(hipMalloc((void**) &c->d_queue, sizeof(unsigned int) * 4));
+  XT_HIP(hipMalloc((void**) &c->d_t0, sizeof(unsigned long long) * std::max<long>(nt, 1)));
+  XT_HIP(hipMalloc((void**) &c->d_t1, sizeof(unsigned long long) * std::max<long>(nt, 1)));
+  XT_HIP(hipMemcpyAsync(c->d_tasks, P.dev.data(), sizeof(xt::TaskDev) * nt, hipMemcpyHostToDevice,
+                        c->stream));
+  XT_HIP(hipMemcpyAsync(c->d_unit_task, unit_task.data(), sizeof(int) * unit_task.size(),
+                        hipMemcpyHostToDevice, c->stream));
+  XT_HIP(hipStreamSynchronize(c->stream));
+  c->dev_tasks_n = nt;
+  return XT_OK;
+}
+
+static std::vector<int> unit_order(const Plan& P, int schedule) {
+  const long       nt = (long) P.dev.size();
+  std::vector<int> task_order(nt);
+  std::iota(task_order.begin(), task_order.end(), 0);
+  if(schedule != XT_SCHED_DRIVER)
+    std::stable_sort(task_order.begin(), task_order.end(),
+                     [&](int a, int b) { return P.dev[a].flops > P.dev[b].flops; });
+  std::vector<int> order;
+  order.reserve(P.n_units);
+  for(int t: task_order)
+    for(int u = 0; u < P.dev[t].n_units; u++) order.push_back(P.dev[t].unit_begin + u);
+  return order;
+}
+
+// Launches the persistent kernel over the units of `tasks_sel` (all tasks if
+// null) in the order given by the schedule, then the fixed-order reductions.
+static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_sel) {
+  const Plan& P  = c->plan;
+  const long  nt = (long) P.dev.size();
+  std::vector<int> order;
+  if(tasks_sel) {
+    std::vector<int> sel = *tasks_sel;
+    if(o.schedule != XT_SCHED_DRIVER)
+      std::stable_sort(sel.begin(), sel.end(),
+                       [&](int a, int b) { return P.dev[a].flops > P.dev[b].flops; });
+    for(int t: sel)
+      for(int u = 0; u < P.dev[t].n_units; u++) order.push_back(P.dev[t].unit_begin + u);
+  }
+  else order = unit_order(P, o.schedule);
+  hipStream_t st = c->stream;
+  if(!order.empty())
+    XT_HIP(hipMemcpyAsync(c->d_order, order.data(), sizeof(int) * order.size(),
+                          hipMemcpyHostToDevice, st));
+  XT_HIP(hipMemsetAsync(c->d_queue, 0, sizeof(unsigned int) * 4, st));
+  XT_HIP(hipMemsetAsync(c->d_unit_e, 0, sizeof(double) * 2 * std::max<long>(P.n_units, 1), st));
+  const bool timing = o.record_task_time != 0;
+  if(timing) {
+    XT_HIP(hipMemsetAsync(c->d_t0, 0xff, sizeof(unsigned long long) * nt, st));
+    XT_HIP(hipMemsetAsync(c->d_t1, 0, sizeof(unsigned long long) * nt, st));
+  }
+  xt::SpatialDev S;
+  S.O    = P.O;
+  S.V    = P.V;
+  S.eo   = c->eo;
+  S.ev   = c->ev;
+  S.t1   = c->t1;
+  S.t2h  = c->t2h;
+  S.t2p  = c->t2p;
+  S.ovk  = c->ovk;
+  S.xv   = c->xv;
+  S.vvoo = c->vvoo;
+  xt::RunDev R;
+  R.tasks     = c->d_tasks;
+  R.unit_task = c->d_unit_task;
+  R.order     = c->d_order;
+  R.n_units   = (int) order.size();
+  R.n_tasks   = (int) nt;
+  R.unit_e    = c->d_unit_e;
+  R.queue     = c->d_queue;
+  R.task_t0   = timing ? c->d_t0 : nullptr;
+  R.task_t1   = timing ? c->d_t1 : nullptr;
+  int grid    = o.workgroups > 0 ? o.workgroups : c->n_cu * 2;
+  grid        = (int) std::min<long>(grid, std::max<long>((long) order.size(), 1));
+  XT_HIP(hipEventRecord(c->ev0, st));
+  int err = xt::launch_spatial_fp64(S, R, grid, st);
+  if(err) return fail(XT_EHIP, std::string("launch: ") + hipGetErrorString((hipError_t) err));
+  XT_HIP(hipEventRecord(c->ev1, st));
+  err = xt::launch_reduce(c->d_tasks, (int) nt, c->d_unit_e, c->d_task_e, c->d_e45, st);
+  if(err) return fail(XT_EHIP, std::string("reduce: ") + hipGetErrorString((hipError_t) err));
+  return XT_OK;
+}
+
+static int check_run_state(xt_ctx* c, const xt_opts& o) {
+  if(c->device < 0) return fail(XT_ENODEV, "xt_run: host-only context");
+  if(!c->have_plan || !c->have_inputs) return fail(XT_ESTATE, "xt_run: tiling or inputs missing");
+  if(o.precision != XT_PREC_FP64)
+    return fail(XT_EINVAL, "xt_run: precision not supported by this build");
+  if(o.schedule < XT_SCHED_DRIVER || o.schedule > XT_SCHED_LEARNED)
+    return fail(XT_EINVAL, "xt_run: bad schedule");
+  XT_HIP(hipSetDevice(c->device));
+  return ensure_plan_dev(c);
+}
+
+static int task_times(xt_ctx* c, long nt, float* task_ms) {
+  std::vector<unsigned long long> t0(nt), t1(nt);
+  XT_HIP(hipMemcpy(t0.data(), c->d_t0, nt * 8, hipMemcpyDeviceToHost));
+  XT_HIP(hipMemcpy(t1.data(), c->d_t1, nt * 8, hipMemcpyDeviceToHost));
+  int rate_khz = 100000;
+  (void) hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device);
+  for(long k = 0; k < nt; k++)
+    task_ms[k] = t1[k] > t0[k] ? (float) ((double) (t1[k] - t0[k]) / rate_khz) : 0.f;
+  return XT_OK;
+}
+
+int xt_run(xt_ctx* c, const xt_opts* opts, double* e45, double* task_partials, float* task_ms) {
+  if(!c || !e45) return fail(XT_EINVAL, "xt_run: null argument");
+  xt_opts o;
+  if(opts) o = *opts;
+  else xt_default_opts(&o);
+  int rc = check_run_state(c, o);
+  if(rc) return rc;
+  const long nt = (long) c->plan.dev.size();
+  if((rc = run_units(c, o, nullptr))) return rc;
+  hipStream_t st = c->stream;
+  XT_HIP(hipMemcpyAsync(e45, c->d_e45, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  if(task_partials)
+    XT_HIP(hipMemcpyAsync(task_partials, c->d_task_e, 2 * nt * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+  XT_HIP(hipStreamSynchronize(st));
+  float ms = 0;
+  XT_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  if(o.record_task_time && task_ms) return task_times(c, nt, task_ms);
+  return XT_OK;
+}
+
+int xt_run_tasks(xt_ctx* c, const xt_opts* opts, long n, const long* which, double* e45,
+                 double* task_partials, float* task_ms) {
+  if(!c || !which || n <= 0 || !e45) return fail(XT_EINVAL, "xt_run_tasks: bad argument");
+  xt_opts o;
+  if(opts) o = *opts;
+  else xt_default_opts(&o);
+  int rc = check_run_state(c, o);
+  if(rc) return rc;
+  const long       nt = (long) c->plan.dev.size();
+  std::vector<int> sel(n);
+  std::vector<char> seen(nt, 0);
+  for(long i = 0; i < n; i++) {
+    if(which[i] < 0 || which[i] >= nt) return fail(XT_EINVAL, "xt_run_tasks: task out of range");
+    if(seen[which[i]]) return fail(XT_EINVAL, "xt_run_tasks: repeated task");
+    seen[which[i]] = 1;
+    sel[i]         = (int) which[i];
+  }
+  if((rc = run_units(c, o, &sel))) return rc;
+  std::vector<double> te(2 * nt);
+  XT_HIP(hipMemcpyAsync(te.data(), c->d_task_e, 2 * nt * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  XT_HIP(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  XT_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  e45[0] = e45[1] = 0.0;
+  for(long i = 0; i < n; i++) {  // listed order
+    e45[0] += te[2 * which[i]];
+    e45[1] += te[2 * which[i] + 1];
+    if(task_partials) {
+      task_partials[2 * i]     = te[2 * which[i]];
+      task_partials[2 * i + 1] = te[2 * which[i] + 1];
+    }
+  }
+  if(o.record_task_time && task_ms) {
+    std::vector<float> all(nt);
+    if((rc = task_times(c, nt, all.data()))) return rc;
+    for(long i = 0; i < n; i++) task_ms[i] = all[which[i]];
+  }
+  return XT_OK;
+}
+
+int xt_task_codes(xt_ctx* c, long task, uint64_t* codes) {
+  if(!c || !codes) return fail(XT_EINVAL, "xt_task_codes: null argument");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_task_codes: no tiling");
+  if(task < 0 || task >= (long) c->plan.dev.size())
+    return fail(XT_EINVAL, "xt_task_codes: task out of range");
+  codes[0] = c->plan.dev[task].d1code;
+  codes[1] = c->plan.dev[task].d2code;
+  codes[2] = c->plan.dev[task].s1code;
+  return XT_OK;
+}
+
+int xt_last_kernel_ms(xt_ctx* c, double* ms) {
+  if(!c || !ms) return fail(XT_EINVAL, "xt_last_kernel_ms: null argument");
+  *ms = c->last_ms;
+  return XT_OK;
+}
+
+// ------------------------------------------------------ per-task drop-in
+namespace {
+constexpr int kMaxNoab = 50, kMaxNvab = 140;  // reference limits (nontc:56-57)
+constexpr int kMetaInts = 16 * kMaxNoab + 16 * kMaxNvab + 15;
+constexpr int kRing     = 4;
+
+struct PackedState {
+  int        device = -1;
+  int*       pinned = nullptr;
+  int*       dmeta  = nullptr;
+  hipEvent_t ev[kRing] = {};
+  int        slot      = 0;
+  ~PackedState() {
+    // Resources live until thread exit; the runtime may already be torn down
+    // at process exit, so errors are ignored.
+    if(device >= 0) {
+      for(auto& e: ev)
+        if(e) (void) hipEventDestroy(e);
+      if(pinned) (void) hipHostFree(pinned);
+      if(dmeta) (void) hipFree(dmeta);
+    }
+  }
+};
+thread_local PackedState g_packed;
+}  // namespace
+
+int xt_fully_fused_ccsd_t_gpu(void* stream, size_t num_blocks, size_t h1, size_t h2, size_t h3,
+                              size_t p4, size_t p5, size_t p6, const double* d1_t2,
+                              const double* d1_v2, const double* d2_t2, const double* d2_v2,
+                              const double* s1_t1, const double* s1_v2, const int* d1_size,
+                              const int* d1_exec, const int* d2_size, const int* d2_exec,
+                              const int* s1_size, const int* s1_exec, size_t noab,
+                              size_t max_d1_t2, size_t max_d1_v2, size_t nvab, size_t max_d2_t2,
+                              size_t max_d2_v2, size_t max_s1_t1, size_t max_s1_v2,
+                              const double* evl_h1, const double* evl_h2, const double* evl_h3,
+                              const double* evl_p4, const double* evl_p5, const double* evl_p6,
+                              double* partial, void* done_copy) {
+  if(!d1_t2 || !d1_v2 || !d2_t2 || !d2_v2 || !s1_t1 || !s1_v2 || !d1_size || !d1_exec ||
+     !d2_size || !d2_exec || !s1_size || !s1_exec || !evl_h1 || !evl_h2 || !evl_h3 || !evl_p4 ||
+     !evl_p5 || !evl_p6 || !partial)
+    return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: null argument");
+  if(noab == 0 || nvab == 0 || noab > (size_t) kMaxNoab || nvab > (size_t) kMaxNvab)
+    return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: noab/nvab outside 1..50 / 1..140");
+  const size_t d[6] = {h1, h2, h3, p4, p5, p6};
+  size_t       nb   = 1;
+  for(size_t x: d) {
+    if(x == 0 || x > 4096) return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: bad tile size");
+    nb *= (x + 3) / 4;
+  }
+  if(nb != num_blocks)
+    return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: num_blocks != prod(ceil(dim/4))");
+  for(int i = 0; i < 6; i++)
+    if((size_t) s1_size[i] != d[i])
+      return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: s1_size disagrees with tile sizes");
+  PackedState& S = g_packed;
+  int          dev;
+  XT_HIP(hipGetDevice(&dev));
+  if(S.device != dev) {
+    if(S.device >= 0) return fail(XT_ESTATE, "xt_fully_fused_ccsd_t_gpu: device changed on thread");
+    XT_HIP(hipHostMalloc((void**) &S.pinned, sizeof(int) * kMetaInts * kRing));
+    XT_HIP(hipMalloc((void**) &S.dmeta, sizeof(int) * kMetaInts * kRing));
+    for(auto& e: S.ev) XT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    S.device = dev;
+  }
+  hipStream_t st   = (hipStream_t) stream;
+  const int   slot = S.slot;
+  S.slot           = (S.slot + 1) % kRing;
+  XT_HIP(hipEventSynchronize(S.ev[slot]));  // previous use of this ring slot finished
+  int* hm = S.pinned + slot * kMetaInts;
+  int* m  = hm;
+  auto put = [&](const int* src, size_t n) {
+    std::memcpy(m, src, n * sizeof(int));
+    m += n;
+  };
+  put(d1_size, 7 * noab);
+  put(d1_exec, 9 * noab);
+  put(d2_size, 7 * nvab);
+  put(d2_exec, 9 * nvab);
+  put(s1_size, 6);
+  put(s1_exec, 9);
+  for(size_t b = 0; b < noab; b++)
+    if(d1_size[7 * b + 3] <= 0 || d1_size[7 * b + 3] > 4096)
+      return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: bad h7 tile size");
+  for(size_t b = 0; b < nvab; b++)
+    if(d2_size[7 * b + 6] <= 0 || d2_size[7 * b + 6] > 4096)
+      return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: bad p7 tile size");
+  int* dm = S.dmeta + slot * kMetaInts;
+  XT_HIP(hipMemcpyAsync(dm, hm, (size_t) (m - hm) * sizeof(int), hipMemcpyHostToDevice, st));
+  if(done_copy) XT_HIP(hipEventRecord((hipEvent_t) done_copy, st));
+  xt::PackedArgs A;
+  for(int i = 0; i < 6; i++) A.dim[i] = (int) d[i];
+  A.d1_t2      = d1_t2;
+  A.d1_v2      = d1_v2;
+  A.d2_t2      = d2_t2;
+  A.d2_v2      = d2_v2;
+  A.s1_t1      = s1_t1;
+  A.s1_v2      = s1_v2;
+  A.meta       = dm;
+  A.noab       = (int) noab;
+  A.nvab       = (int) nvab;
+  A.max_d1_t2  = (long) max_d1_t2;
+  A.max_d1_v2  = (long) max_d1_v2;
+  A.max_d2_t2  = (long) max_d2_t2;
+  A.max_d2_v2  = (long) max_d2_v2;
+  A.max_s1_t1  = (long) max_s1_t1;
+  A.max_s1_v2  = (long) max_s1_v2;
+  A.evl[0]     = evl_h1;
+  A.evl[1]     = evl_h2;
+  A.evl[2]     = evl_h3;
+  A.evl[3]     = evl_p4;
+  A.evl[4]     = evl_p5;
+  A.evl[5]     = evl_p6;
+  A.partial    = partial;
+  A.num_blocks = (long) num_blocks;
+  int err      = xt::launch_packed_fp64(A, st);
+  if(err) return fail(XT_EHIP, std::string("packed launch: ") + hipGetErrorString((hipError_t) err));
+  XT_HIP(hipEventRecord(S.ev[slot], st));
+  return XT_OK;
+}
+
+}  // extern "C"

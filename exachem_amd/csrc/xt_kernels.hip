@@ -1,0 +1,644 @@
+// gfx950 kernels of the closed-shell CCSD(T) triples energy.
+//
+// Work decomposition (shared by the HBM-resident path and the per-task
+// packed-buffer drop-in):
+//   * a task (h1b,h2b,h3b,p4b,p5b,p6b) is cut into 4x4x4x4x4x4 sub-tiles of
+//     the triples tile t3[h1,h2,h3,p4,p5,p6]; one 256-thread workgroup (4
+//     wave64) owns one sub-tile at a time and keeps it in LDS (never in HBM);
+//   * every d1 and d2 equation of the reference (ccsd_t_all_fused_cpu.hpp:
+//     173-270, 331-428) is a GEMM into one of 9 bipartitions of the six
+//     t3 indices: M = {single hole hs, particle pair (pA,pB)} and
+//     N = {hole pair (hA,hB), single particle ps}.  Each bipartition g is
+//     fed by exactly one d1 equation (K = h7) and one d2 equation (K = p7),
+//     so the two run back to back into one 64x64 FP64 accumulator held in
+//     MFMA registers (v_mfma_f64_16x16x4f64, 2x2 tiles per wave) and the
+//     result is added once into the LDS t3 tile;
+//   * operands are staged through LDS in K-chunks of 16 with a register
+//     prefetch of the next chunk;
+//   * s1 (ccsd_t_all_fused_cpu.hpp:472-578) and the denominator-weighted
+//     energy (586-619) run on the LDS tile; the 256 partials are reduced
+//     with wave64 shuffles and a fixed-order cross-wave sum.
+#include <hip/hip_runtime.h>
+
+#include "xt_device.h"
+
+namespace xt {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int NT      = 256;       // threads per workgroup
+constexpr int KC      = 16;        // K chunk staged per LDS round
+constexpr int LDK     = KC + 1;    // padded row stride (doubles) of staged operands
+constexpr int T3N     = 4096;      // 4^6 sub-tile
+constexpr int S1_T1   = 9 * 16;    // s1 T1 blocks in LDS
+constexpr int STAGE_D = S1_T1 + 9 * 256;  // staging union, >= 2*64*LDK
+
+struct __align__(16) Smem {
+  double t3[T3N];
+  double stage[STAGE_D];
+  double eps[24];
+  double red[8];
+};
+
+__device__ __forceinline__ int cdiv(int a, int b) { return (a + b - 1) / b; }
+// parity sign of the single index at position x (0,1,2) within its triple:
+// (x, others ascending) is even for x=0,2 and odd for x=1.
+__device__ __forceinline__ double sgn_single(int x) { return x == 1 ? -1.0 : 1.0; }
+
+__device__ __forceinline__ void pair_of(int s, int& a, int& b) {
+  a = (s == 0) ? 1 : 0;
+  b = (s == 2) ? 1 : 2;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for(int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum of a per-thread value over the workgroup in a fixed order; result valid
+// in thread 0.
+__device__ __forceinline__ void block_sum2(Smem& sm, double& a, double& b) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if(lane == 0) {
+    sm.red[wave]     = a;
+    sm.red[4 + wave] = b;
+  }
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    a = ((sm.red[0] + sm.red[1]) + sm.red[2]) + sm.red[3];
+    b = ((sm.red[4] + sm.red[5]) + sm.red[6]) + sm.red[7];
+  }
+}
+
+// Index of t3 element (m, n) of bipartition g in the canonical LDS layout
+// e = sum_pos idx[pos] << 2*pos (pos 0..2 = h1..h3, 3..5 = p4..p6).
+__device__ __forceinline__ int emap(int hs, int hA, int hB, int ps, int pA, int pB, int m, int n) {
+  return ((m & 3) << (2 * hs)) | (((m >> 2) & 3) << (2 * (3 + pA))) | ((m >> 4) << (2 * (3 + pB))) |
+         ((n & 3) << (2 * hA)) | (((n >> 2) & 3) << (2 * hB)) | ((n >> 4) << (2 * (3 + ps)));
+}
+
+// The 64x64xKC MFMA step of one staged chunk: wave w computes rows
+// 32*(w>>1).. and columns 32*(w&1).. as 2x2 16x16 tiles.
+__device__ __forceinline__ void mfma_chunk(const double* __restrict__ As,
+                                           const double* __restrict__ Bs, v4d (&acc)[2][2]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rb = (wave >> 1) * 32, cb = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for(int kk = 0; kk < KC / 4; kk++) {
+    const int k  = 4 * kk + lk;
+    double    a0 = As[(rb + lr) * LDK + k];
+    double    a1 = As[(rb + 16 + lr) * LDK + k];
+    double    b0 = Bs[(cb + lr) * LDK + k];
+    double    b1 = Bs[(cb + 16 + lr) * LDK + k];
+    acc[0][0]    = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1]    = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0]    = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1]    = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// Adds the accumulator of bipartition g into the LDS t3 tile.  D layout of
+// v_mfma_f64_16x16x4f64: register r of lane l holds row 4*r + (l>>4), column
+// l&15 (the four row groups are interleaved, unlike the 16-bit forms).
+__device__ __forceinline__ void acc_to_t3(Smem& sm, int g, const v4d (&acc)[2][2]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rb = (wave >> 1) * 32, cb = (wave & 1) * 32;
+  const int hs = g / 3, ps = g % 3;
+  int       hA, hB, pA, pB;
+  pair_of(hs, hA, hB);
+  pair_of(ps, pA, pB);
+#pragma unroll
+  for(int i = 0; i < 2; i++)
+#pragma unroll
+    for(int j = 0; j < 2; j++)
+#pragma unroll
+      for(int r = 0; r < 4; r++) {
+        const int m = rb + 16 * i + 4 * r + (lane >> 4);
+        const int n = cb + 16 * j + (lane & 15);
+        sm.t3[emap(hs, hA, hB, ps, pA, pB, m, n)] += acc[i][j][r];
+      }
+}
+
+// Energy of the finished sub-tile: t3d in LDS, s1 operands staged in LDS,
+// denominators from sm.eps.  Returns (E4, E5) partial sums in thread 0.
+//   T1s[s][pa][hi] (16 per slot), V1s[s][hk + 4 hj + 16 pc + 64 pb] (256 per slot)
+__device__ __forceinline__ void subtile_energy(Smem& sm, const int* sub, const int* dim,
+                                               unsigned s1mask, double& e4, double& e5) {
+  const double* T1s = sm.stage;
+  const double* V1s = sm.stage + S1_T1;
+  e4 = 0.0;
+  e5 = 0.0;
+#pragma unroll 1
+  for(int r = 0; r < T3N / NT; r++) {
+    const int e = threadIdx.x + NT * r;
+    int       idx[6];
+    bool      ok = true;
+#pragma unroll
+    for(int p = 0; p < 6; p++) {
+      idx[p] = (e >> (2 * p)) & 3;
+      ok     = ok && (4 * sub[p] + idx[p] < dim[p]);
+    }
+    if(!ok) continue;
+    const double d = sm.t3[e];
+    double       s = 0.0;
+#pragma unroll
+    for(int slot = 0; slot < 9; slot++) {
+      if(!((s1mask >> slot) & 1)) continue;
+      const int pa = slot / 3, hi = slot % 3;
+      int       hj, hk, pb, pc;
+      pair_of(hi, hj, hk);
+      pair_of(pa, pb, pc);
+      s += T1s[slot * 16 + idx[3 + pa] * 4 + idx[hi]] *
+           V1s[slot * 256 + idx[hk] + 4 * idx[hj] + 16 * idx[3 + pc] + 64 * idx[3 + pb]];
+    }
+    // denominator in the reference's order (ccsd_t_all_fused_cpu.hpp:612-614)
+    const double D = sm.eps[8 + idx[2]] + sm.eps[4 + idx[1]] + sm.eps[idx[0]] -
+                     sm.eps[20 + idx[5]] - sm.eps[16 + idx[4]] - sm.eps[12 + idx[3]];
+    e4 += d * d / D;
+    e5 += d * (d + s) / D;
+  }
+  block_sum2(sm, e4, e5);
+}
+
+// ---------------------------------------------------------------------------
+// HBM-resident path: operands are formed on the fly from spatial closed-shell
+// tensors (spin-orbital antisymmetrised values of <pq||rs> and T2, exactly the
+// numbers the reference's gathers would copy out of TAMM blocks).
+// ---------------------------------------------------------------------------
+struct RowPtrs {
+  const double* p1;
+  const double* p2;
+  double        c1, c2;
+};
+
+__device__ __forceinline__ void fetch4(const RowPtrs& rp, int k0, int K, double* x, double* y) {
+#pragma unroll
+  for(int j = 0; j < 4; j++) {
+    const int k = k0 + j;
+    x[j]        = (rp.c1 != 0.0 && k < K) ? rp.p1[k] : 0.0;
+    y[j]        = (rp.c2 != 0.0 && k < K) ? rp.p2[k] : 0.0;
+  }
+}
+
+__device__ void spatial_subtile(Smem& sm, const SpatialDev& S, const TaskDev& T, const int* sub,
+                                double& e4, double& e5) {
+  const int tid = threadIdx.x;
+  const int O = S.O, V = S.V;
+  __syncthreads();  // previous sub-tile's energy phase is done with LDS
+  for(int e = tid; e < T3N; e += NT) sm.t3[e] = 0.0;
+  if(tid < 24) {
+    const int p = tid >> 2, i = tid & 3, g = 4 * sub[p] + i;
+    sm.eps[tid] = g < T.dim[p] ? (p < 3 ? S.eo : S.ev)[T.off[p] + g] : 0.0;
+  }
+  __syncthreads();
+
+  const int row = tid >> 2, kq = tid & 3;
+  double*   As  = sm.stage;
+  double*   Bs  = sm.stage + 64 * LDK;
+
+#pragma unroll 1
+  for(int g = 0; g < 9; g++) {
+    const unsigned c1 = (unsigned) (T.d1code >> (6 * g)) & 63u;
+    const unsigned c2 = (unsigned) (T.d2code >> (6 * g)) & 63u;
+    const int      n1 = (c1 & 1) ? cdiv(O, KC) : 0;
+    const int      n2 = (c2 & 1) ? cdiv(V, KC) : 0;
+    if(n1 + n2 == 0) continue;
+    const int hs = g / 3, ps = g % 3;
+    int       hA, hB, pA, pB;
+    pair_of(hs, hA, hB);
+    pair_of(ps, pA, pB);
+    // this thread's M row (hs, pA, pB) and N row (hA, hB, ps)
+    const int gHS = 4 * sub[hs] + (row & 3), gPA = 4 * sub[3 + pA] + ((row >> 2) & 3),
+              gPB = 4 * sub[3 + pB] + (row >> 4);
+    const int gHA = 4 * sub[hA] + (row & 3), gHB = 4 * sub[hB] + ((row >> 2) & 3),
+              gPS = 4 * sub[3 + ps] + (row >> 4);
+    const bool mok = gHS < T.dim[hs] && gPA < T.dim[3 + pA] && gPB < T.dim[3 + pB];
+    const bool nok = gHA < T.dim[hA] && gHB < T.dim[hB] && gPS < T.dim[3 + ps];
+    const size_t HS = T.off[hs] + (mok ? gHS : 0), PA = T.off[3 + pA] + (mok ? gPA : 0),
+                 PB = T.off[3 + pB] + (mok ? gPB : 0);
+    const size_t HA = T.off[hA] + (nok ? gHA : 0), HB = T.off[hB] + (nok ? gHB : 0),
+                 PS = T.off[3 + ps] + (nok ? gPS : 0);
+    RowPtrs a1r, b1r, a2r, b2r;
+    {
+      const double s = (c1 & 2) ? -1.0 : 1.0;
+      a1r.p1 = S.t2h + ((PA * V + PB) * O + HS) * O;  // t(pA,pB,hs,h7)
+      a1r.p2 = S.t2h + ((PB * V + PA) * O + HS) * O;  // t(pB,pA,hs,h7) = t(pA,pB,h7,hs)
+      a1r.c1 = (mok && (c1 & 4)) ? s : 0.0;
+      a1r.c2 = (mok && (c1 & 8)) ? -s : 0.0;
+      b1r.p1 = S.ovk + ((HB * V + PS) * O + HA) * O;  // (hA h7|hB ps)
+      b1r.p2 = S.ovk + ((HA * V + PS) * O + HB) * O;  // (hB h7|hA ps)
+      b1r.c1 = (nok && (c1 & 16)) ? 1.0 : 0.0;
+      b1r.c2 = (nok && (c1 & 32)) ? -1.0 : 0.0;
+    }
+    {
+      const double s = (c2 & 2) ? -1.0 : 1.0;
+      a2r.p1 = S.xv + ((PB * V + PA) * O + HS) * V;  // (hs pA|p7 pB)
+      a2r.p2 = S.xv + ((PA * V + PB) * O + HS) * V;  // (hs pB|p7 pA)
+      a2r.c1 = (mok && (c2 & 4)) ? s : 0.0;
+      a2r.c2 = (mok && (c2 & 8)) ? -s : 0.0;
+      b2r.p1 = S.t2p + ((HA * O + HB) * V + PS) * V;  // t(ps,p7,hA,hB)
+      b2r.p2 = S.t2p + ((HB * O + HA) * V + PS) * V;  // t(p7,ps,hA,hB)
+      b2r.c1 = (nok && (c2 & 16)) ? 1.0 : 0.0;
+      b2r.c2 = (nok && (c2 & 32)) ? -1.0 : 0.0;
+    }
+
+    v4d acc[2][2];
+#pragma unroll
+    for(int i = 0; i < 2; i++)
+#pragma unroll
+      for(int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    double    ax[4], ay[4], bx[4], by[4];
+    double    ca1, ca2, cb1, cb2;
+    const int nc = n1 + n2;
+    auto      fetch = [&](int c) {
+      if(c < n1) {
+        fetch4(a1r, c * KC + 4 * kq, O, ax, ay);
+        fetch4(b1r, c * KC + 4 * kq, O, bx, by);
+      }
+      else {
+        fetch4(a2r, (c - n1) * KC + 4 * kq, V, ax, ay);
+        fetch4(b2r, (c - n1) * KC + 4 * kq, V, bx, by);
+      }
+    };
+    fetch(0);
+#pragma unroll 1
+    for(int c = 0; c < nc; c++) {
+      if(c < n1) {
+        ca1 = a1r.c1; ca2 = a1r.c2; cb1 = b1r.c1; cb2 = b1r.c2;
+      }
+      else {
+        ca1 = a2r.c1; ca2 = a2r.c2; cb1 = b2r.c1; cb2 = b2r.c2;
+      }
+      __syncthreads();
+#pragma unroll
+      for(int j = 0; j < 4; j++) {
+        As[row * LDK + 4 * kq + j] = ca1 * ax[j] + ca2 * ay[j];
+        Bs[row * LDK + 4 * kq + j] = cb1 * bx[j] + cb2 * by[j];
+      }
+      __syncthreads();
+      if(c + 1 < nc) fetch(c + 1);
+      mfma_chunk(As, Bs, acc);
+    }
+    acc_to_t3(sm, g, acc);
+  }
+
+  // ---- s1 operands: T1so(pa,hi) and <hk hj||pc pb> blocks of the sub-tile
+  __syncthreads();
+  double*  T1s    = sm.stage;
+  double*  V1s    = sm.stage + S1_T1;
+  unsigned s1mask = 0;
+#pragma unroll
+  for(int slot = 0; slot < 9; slot++)
+    if((T.s1code >> (4 * slot)) & 1) s1mask |= 1u << slot;
+  for(int x = tid; x < 9 * 16; x += NT) {
+    const int      slot = x >> 4, pai = (x >> 2) & 3, hii = x & 3;
+    const unsigned c    = (unsigned) (T.s1code >> (4 * slot)) & 15u;
+    const int      pa = slot / 3, hi = slot % 3;
+    const int      gp = 4 * sub[3 + pa] + pai, gh = 4 * sub[hi] + hii;
+    double         v  = 0.0;
+    if((c & 1) && gp < T.dim[3 + pa] && gh < T.dim[hi]) {
+      v = S.t1[(size_t) (T.off[3 + pa] + gp) * O + T.off[hi] + gh];
+      if(c & 2) v = -v;
+    }
+    T1s[x] = v;
+  }
+  for(int x = tid; x < 9 * 256; x += NT) {
+    const int      slot = x >> 8, w = x & 255;
+    const unsigned c    = (unsigned) (T.s1code >> (4 * slot)) & 15u;
+    const int      pa = slot / 3, hi = slot % 3;
+    int            hj, hk, pb, pc;
+    pair_of(hi, hj, hk);
+    pair_of(pa, pb, pc);
+    const int gk = 4 * sub[hk] + (w & 3), gj = 4 * sub[hj] + ((w >> 2) & 3),
+              gc = 4 * sub[3 + pc] + ((w >> 4) & 3), gb = 4 * sub[3 + pb] + (w >> 6);
+    double v = 0.0;
+    if((c & 1) && gk < T.dim[hk] && gj < T.dim[hj] && gc < T.dim[3 + pc] && gb < T.dim[3 + pb]) {
+      const size_t K = T.off[hk] + gk, J = T.off[hj] + gj, PC = T.off[3 + pc] + gc,
+                   PBx = T.off[3 + pb] + gb;
+      const double x1 = (c & 4) ? S.vvoo[((PC * V + PBx) * O + K) * O + J] : 0.0;  // (hk pc|hj pb)
+      const double x2 = (c & 8) ? S.vvoo[((PBx * V + PC) * O + K) * O + J] : 0.0;  // (hk pb|hj pc)
+      v               = ((c & 4) && (c & 8)) ? x1 - x2 : ((c & 4) ? x1 : -x2);
+    }
+    V1s[x] = v;
+  }
+  __syncthreads();
+  subtile_energy(sm, sub, T.dim, s1mask, e4, e5);
+}
+
+__device__ __forceinline__ void decode_sub(const TaskDev& T, int q, int* sub) {
+#pragma unroll
+  for(int p = 0; p < 6; p++) {
+    sub[p] = q % T.nsub[p];
+    q /= T.nsub[p];
+  }
+}
+
+__global__ void __launch_bounds__(NT) k_spatial_fp64(SpatialDev S, RunDev R) {
+  __shared__ Smem sm;
+  __shared__ int  s_unit;
+#pragma unroll 1
+  for(;;) {
+    __syncthreads();
+    if(threadIdx.x == 0) {
+      const unsigned t = atomicAdd(R.queue, 1u);
+      s_unit           = (int) t < R.n_units ? (R.order ? R.order[t] : (int) t) : -1;
+    }
+    __syncthreads();
+    const int unit = s_unit;
+    if(unit < 0) break;
+    const int     ti = R.unit_task[unit];
+    const TaskDev T  = R.tasks[ti];
+    if(R.task_t0 && threadIdx.x == 0) {
+      const unsigned long long now = wall_clock64();
+      atomicMin(&R.task_t0[ti], now);
+    }
+    const int q0 = (unit - T.unit_begin) * T.subs_per_unit;
+    const int q1 = min(q0 + T.subs_per_unit, T.n_sub);
+    double    u4 = 0.0, u5 = 0.0;
+#pragma unroll 1
+    for(int q = q0; q < q1; q++) {
+      int sub[6];
+      decode_sub(T, q, sub);
+      double e4, e5;
+      spatial_subtile(sm, S, T, sub, e4, e5);
+      if(threadIdx.x == 0) {
+        u4 += e4;
+        u5 += e5;
+      }
+    }
+    if(threadIdx.x == 0) {
+      R.unit_e[2 * (size_t) unit]     = u4;
+      R.unit_e[2 * (size_t) unit + 1] = u5;
+      if(R.task_t1) atomicMax(&R.task_t1[ti], wall_clock64());
+    }
+  }
+}
+
+// Per-task fixed-order sum of unit partials, scaled by the task factor.
+__global__ void k_reduce_tasks(const TaskDev* tasks, int n_tasks, const double* unit_e,
+                               double* task_e) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= n_tasks) return;
+  const TaskDev& T = tasks[t];
+  double         a = 0.0, b = 0.0;
+  for(int u = T.unit_begin; u < T.unit_begin + T.n_units; u++) {
+    a += unit_e[2 * (size_t) u];
+    b += unit_e[2 * (size_t) u + 1];
+  }
+  task_e[2 * (size_t) t]     = T.factor * a;
+  task_e[2 * (size_t) t + 1] = T.factor * b;
+}
+
+// Fixed-shape sum over tasks in task order (one workgroup of NT threads).
+__global__ void __launch_bounds__(NT) k_reduce_final(const double* task_e, int n, double* e45) {
+  __shared__ double ra[NT], rb[NT];
+  const int         per = (n + NT - 1) / NT;
+  const int         lo = threadIdx.x * per, hi = min(n, lo + per);
+  double            a = 0.0, b = 0.0;
+  for(int t = lo; t < hi; t++) {
+    a += task_e[2 * (size_t) t];
+    b += task_e[2 * (size_t) t + 1];
+  }
+  ra[threadIdx.x] = a;
+  rb[threadIdx.x] = b;
+  __syncthreads();
+  for(int s = NT / 2; s >= 1; s >>= 1) {
+    if((int) threadIdx.x < s) {
+      ra[threadIdx.x] += ra[threadIdx.x + s];
+      rb[threadIdx.x] += rb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) {
+    e45[0] = ra[0];
+    e45[1] = rb[0];
+  }
+}
+
+// out[b*ldo + a] = in[a*lda + b] for a < rows, b < cols, batched over
+// blockIdx.z with strides (in_bs, out_bs).  32x32 LDS tiles.
+__global__ void k_transpose(const double* __restrict__ in, double* __restrict__ out, long rows,
+                            long cols, long lda, long ldo, long in_bs, long out_bs) {
+  __shared__ double tile[32][33];
+  const long z  = blockIdx.z;
+  const long a0 = (long) blockIdx.y * 32, b0 = (long) blockIdx.x * 32;
+  const int  tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+  for(int r = ty; r < 32; r += 8) {
+    const long a = a0 + r, b = b0 + tx;
+    tile[r][tx]  = (a < rows && b < cols) ? in[z * in_bs + a * lda + b] : 0.0;
+  }
+  __syncthreads();
+  for(int r = ty; r < 32; r += 8) {
+    const long b = b0 + r, a = a0 + tx;
+    if(a < rows && b < cols) out[z * out_bs + b * ldo + a] = tile[tx][r];
+  }
+}
+
+static int transpose(const double* in, double* out, long rows, long cols, long lda, long batch,
+                     long in_bs, long out_bs, hipStream_t st) {
+  // grid.y and grid.z are limited to 65535
+  const long max_yz = 65535;
+  for(long z0 = 0; z0 < batch; z0 += max_yz) {
+    const long nz = std::min(max_yz, batch - z0);
+    for(long r0 = 0; r0 < rows; r0 += 32 * max_yz) {
+      const long nr = std::min(32 * max_yz, rows - r0);
+      dim3       grid((unsigned) ((cols + 31) / 32), (unsigned) ((nr + 31) / 32), (unsigned) nz);
+      hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, st, in + z0 * in_bs + r0 * lda,
+                         out + z0 * out_bs + r0, nr, cols, lda, rows, in_bs, out_bs);
+      if(hipGetLastError() != hipSuccess) return (int) hipErrorLaunchFailure;
+    }
+  }
+  return 0;
+}
+
+int launch_relayout(const double* t2_in, const double* ovoo_in, const double* vvvo_in, int O,
+                    int V, double* t2h, double* t2p, double* ovk, double* xv, void* stream) {
+  hipStream_t st = (hipStream_t) stream;
+  const long  o = O, v = V;
+  hipError_t  err;
+  // t2h = t2 as given, ovk = ovoo as given
+  if(t2h != t2_in &&
+     (err = hipMemcpyAsync(t2h, t2_in, sizeof(double) * v * v * o * o, hipMemcpyDeviceToDevice, st)))
+    return (int) err;
+  if(ovk != ovoo_in &&
+     (err = hipMemcpyAsync(ovk, ovoo_in, sizeof(double) * o * v * o * o, hipMemcpyDeviceToDevice, st)))
+    return (int) err;
+  // t2p[i][j][a][b] = t2[a][b][i][j]: transpose of a (V*V) x (O*O) matrix
+  int rc = transpose(t2_in, t2p, v * v, o * o, o * o, 1, 0, 0, st);
+  if(rc) return rc;
+  // xv[c][b][i][a] = vvvo[a][c][b][i]: for each c, transpose rows a (stride V*V*O)
+  // x columns (b,i) (V*O contiguous) into out[c][(b,i)][a].
+  rc = transpose(vvvo_in, xv, v, v * o, v * v * o, v, v * o, v * o * v, st);
+  return rc;
+}
+
+int launch_spatial_fp64(const SpatialDev& S, const RunDev& R, int grid, void* stream) {
+  hipLaunchKernelGGL(k_spatial_fp64, dim3(grid), dim3(NT), 0, (hipStream_t) stream, S, R);
+  return (int) hipGetLastError();
+}
+
+int launch_reduce(const TaskDev* tasks, int n_tasks, const double* unit_e, double* task_e,
+                  double* e45, void* stream) {
+  hipStream_t st = (hipStream_t) stream;
+  hipLaunchKernelGGL(k_reduce_tasks, dim3((n_tasks + 255) / 256), dim3(256), 0, st, tasks,
+                     n_tasks, unit_e, task_e);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(NT), 0, st, task_e, n_tasks, e45);
+  return (int) hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Per-task drop-in: operands come from the reference's packed buffers
+// (ccsd_t_data_{s1,d1,d2}_new layouts, ccsd_t_all_fused_cpu.hpp index
+// arithmetic), one workgroup per 4^6 block, partials per block.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
+  __shared__ Smem sm;
+  const int       tid = threadIdx.x, row = tid >> 2, kq = tid & 3;
+  const int*      dim = A.dim;
+  int             nsub[6], sub[6];
+#pragma unroll
+  for(int p = 0; p < 6; p++) nsub[p] = cdiv(dim[p], 4);
+  {
+    long q = blockIdx.x;
+#pragma unroll
+    for(int p = 0; p < 6; p++) {
+      sub[p] = (int) (q % nsub[p]);
+      q /= nsub[p];
+    }
+  }
+  const int* d1_size = A.meta;
+  const int* d1_exec = d1_size + 7 * A.noab;
+  const int* d2_size = d1_exec + 9 * A.noab;
+  const int* d2_exec = d2_size + 7 * A.nvab;
+  const int* s1_exec = d2_exec + 9 * A.nvab + 6;
+
+  for(int e = tid; e < T3N; e += NT) sm.t3[e] = 0.0;
+  if(tid < 24) {
+    const int p = tid >> 2, i = tid & 3, g = 4 * sub[p] + i;
+    sm.eps[tid] = g < dim[p] ? A.evl[p][g] : 0.0;
+  }
+  __syncthreads();
+  double* As = sm.stage;
+  double* Bs = sm.stage + 64 * LDK;
+
+#pragma unroll 1
+  for(int g = 0; g < 9; g++) {
+    const int hs = g / 3, ps = g % 3;
+    int       hA, hB, pA, pB;
+    pair_of(hs, hA, hB);
+    pair_of(ps, pA, pB);
+    const int    s1d = 3 * ((ps + 1) % 3) + hs;  // reference d1 slot of this bipartition
+    const int    s2d = 3 * ps + (hs + 1) % 3;    // reference d2 slot
+    const double sg  = -sgn_single(hs) * sgn_single(ps);
+    const int gHS = 4 * sub[hs] + (row & 3), gPA = 4 * sub[3 + pA] + ((row >> 2) & 3),
+              gPB = 4 * sub[3 + pB] + (row >> 4);
+    const int gHA = 4 * sub[hA] + (row & 3), gHB = 4 * sub[hB] + ((row >> 2) & 3),
+              gPS = 4 * sub[3 + ps] + (row >> 4);
+    const bool mok = gHS < dim[hs] && gPA < dim[3 + pA] && gPB < dim[3 + pB];
+    const bool nok = gHA < dim[hA] && gHB < dim[hB] && gPS < dim[3 + ps];
+    const long DHS = dim[hs], DPA = dim[3 + pA], DPB = dim[3 + pB];
+    const long DHA = dim[hA], DHB = dim[hB], DPS = dim[3 + ps];
+    v4d        acc[2][2];
+#pragma unroll
+    for(int i = 0; i < 2; i++)
+#pragma unroll
+      for(int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    bool any = false;
+    // d1 tiles (h7b), then d2 tiles (p7b)
+#pragma unroll 1
+    for(int fam = 0; fam < 2; fam++) {
+      const int ntile = fam == 0 ? A.noab : A.nvab;
+#pragma unroll 1
+      for(int tb = 0; tb < ntile; tb++) {
+        const int ex = fam == 0 ? d1_exec[s1d + 9 * tb] : d2_exec[s2d + 9 * tb];
+        if(ex < 0) continue;
+        any          = true;
+        const int K  = fam == 0 ? d1_size[3 + 7 * tb] : d2_size[6 + 7 * tb];
+        const double* abase;
+        const double* bbase;
+        long          astr = 1, bstr = 1;  // stride of k
+        if(fam == 0) {
+          abase = A.d1_t2 + A.max_d1_t2 * (long) ex + (gPA + (gPB + gHS * DPB) * DPA) * (long) K;
+          bbase = A.d1_v2 + A.max_d1_v2 * (long) ex + (gHB + (gHA + gPS * DHA) * DHB);
+          bstr  = DPS * DHA * DHB;
+        }
+        else {
+          abase = A.d2_v2 + A.max_d2_v2 * (long) ex + (gHS + (gPB + gPA * DPB) * DHS) * (long) K;
+          bbase = A.d2_t2 + A.max_d2_t2 * (long) ex + (gPS + (gHA + gHB * DHA) * DPS) * (long) K;
+        }
+        const int nc = cdiv(K, KC);
+#pragma unroll 1
+        for(int c = 0; c < nc; c++) {
+          double av[4], bv[4];
+#pragma unroll
+          for(int j = 0; j < 4; j++) {
+            const int k = c * KC + 4 * kq + j;
+            av[j]       = (mok && k < K) ? sg * abase[k * astr] : 0.0;
+            bv[j]       = (nok && k < K) ? bbase[k * bstr] : 0.0;
+          }
+          __syncthreads();
+#pragma unroll
+          for(int j = 0; j < 4; j++) {
+            As[row * LDK + 4 * kq + j] = av[j];
+            Bs[row * LDK + 4 * kq + j] = bv[j];
+          }
+          __syncthreads();
+          mfma_chunk(As, Bs, acc);
+        }
+      }
+    }
+    if(any) acc_to_t3(sm, g, acc);
+  }
+
+  // s1 operands from the packed buffers: t1 [hi][pa], v2 [pb][pc][hj][hk]
+  __syncthreads();
+  double*  T1s    = sm.stage;
+  double*  V1s    = sm.stage + S1_T1;
+  unsigned s1mask = 0;
+  for(int slot = 0; slot < 9; slot++)
+    if(s1_exec[slot] >= 0) s1mask |= 1u << slot;
+  for(int x = tid; x < 9 * 16; x += NT) {
+    const int slot = x >> 4, pai = (x >> 2) & 3, hii = x & 3;
+    const int pa = slot / 3, hi = slot % 3;
+    const int gp = 4 * sub[3 + pa] + pai, gh = 4 * sub[hi] + hii;
+    double    v  = 0.0;
+    if(s1_exec[slot] >= 0 && gp < dim[3 + pa] && gh < dim[hi])
+      v = sgn_single(pa) * sgn_single(hi) *
+          A.s1_t1[A.max_s1_t1 * (long) s1_exec[slot] + gp + (long) gh * dim[3 + pa]];
+    T1s[x] = v;
+  }
+  for(int x = tid; x < 9 * 256; x += NT) {
+    const int slot = x >> 8, w = x & 255;
+    const int pa = slot / 3, hi = slot % 3;
+    int       hj, hk, pb, pc;
+    pair_of(hi, hj, hk);
+    pair_of(pa, pb, pc);
+    const int gk = 4 * sub[hk] + (w & 3), gj = 4 * sub[hj] + ((w >> 2) & 3),
+              gc = 4 * sub[3 + pc] + ((w >> 4) & 3), gb = 4 * sub[3 + pb] + (w >> 6);
+    double v = 0.0;
+    if(s1_exec[slot] >= 0 && gk < dim[hk] && gj < dim[hj] && gc < dim[3 + pc] && gb < dim[3 + pb])
+      v = A.s1_v2[A.max_s1_v2 * (long) s1_exec[slot] + gk +
+                  (gj + (gc + (long) gb * dim[3 + pc]) * dim[hj]) * dim[hk]];
+    V1s[x] = v;
+  }
+  __syncthreads();
+  double e4, e5;
+  subtile_energy(sm, sub, dim, s1mask, e4, e5);
+  if(tid == 0) {
+    A.partial[blockIdx.x]                = e4;
+    A.partial[blockIdx.x + A.num_blocks] = e5;
+  }
+}
+
+int launch_packed_fp64(const PackedArgs& A, void* stream) {
+  hipLaunchKernelGGL(k_packed_fp64, dim3((unsigned) A.num_blocks), dim3(NT), 0,
+                     (hipStream_t) stream, A);
+  return (int) hipGetLastError();
+}
+
+}  // namespace xt

@@ -1,0 +1,213 @@
+"""Host-side front end of the fused (T) path.
+
+``CCSDTriples`` mirrors the call sequence of ExaChem's
+``CCSD_T_Fused_Driver<T>::execute`` (exachem/cc/ccsd_t/ccsd_t_fused_driver.hpp:
+91-546): a tiled MO space with k_spin (ccsd_t.cpp:245-249), T1/T2 and V2
+inputs, ``is_restricted``; it returns (E[4], E[5]) = (energy1, energy2) of
+the reference's tuple, i.e. the [T] and (T) corrections.  All work happens in
+the native library; errors surface as ``XtError`` carrying the library's
+message (the reference would ``exit(100)`` instead, ccsd_t_common.hpp:35-41).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import PREC_FP64, SCHED_DRIVER, XtOpts, check, lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def uniform_tiles(n: int, ts: int):
+    """Tiling of one spin part as setupMOIS does it (full tiles, remainder last;
+    exachem/cholesky/cholesky_2e.cpp:226-277)."""
+    return [ts] * (n // ts) + ([n % ts] if n % ts else [])
+
+
+def uneven_tiles(n: int, lo: int, hi: int, seed: int):
+    """Seeded uneven tiling: sizes drawn uniformly in [lo, hi]; the last tiles
+    are adjusted so that every tile stays in [lo, hi] whenever n >= lo."""
+    rng = np.random.default_rng(seed)
+    out, rem = [], n
+    while rem > 0:
+        if rem <= hi:
+            t = rem
+        else:
+            t = int(rng.integers(lo, hi + 1))
+            if 0 < rem - t < lo:
+                t = rem - lo
+        out.append(t)
+        rem -= t
+    return out
+
+
+@dataclass
+class Tiling:
+    O: int
+    V: int
+    occ: list = field(default_factory=list)
+    vir: list = field(default_factory=list)
+
+    @classmethod
+    def uniform(cls, O, V, occ_ts, vir_ts):
+        return cls(O, V, uniform_tiles(O, occ_ts), uniform_tiles(V, vir_ts))
+
+    @classmethod
+    def uneven(cls, O, V, occ_rng=(6, 20), vir_rng=(16, 48), seed=0):
+        return cls(O, V, uneven_tiles(O, *occ_rng, seed), uneven_tiles(V, *vir_rng, seed + 1))
+
+    def spin_orbital(self):
+        """(tiles, k_spin) of |occ_a|occ_b|virt_a|virt_b| (beta mirrors alpha)."""
+        tiles = self.occ + self.occ + self.vir + self.vir
+        no, nv = len(self.occ), len(self.vir)
+        spins = [1] * no + [2] * no + [1] * nv + [2] * nv
+        return tiles, spins
+
+    def so_energies(self, eo, ev):
+        """k_evl_sorted of the reference (ccsd_t.cpp:209) in spin-orbital tile order."""
+        return np.concatenate([eo, eo, ev, ev])
+
+
+def synth_closed_shell(O, V, seed, sigma=0.05, t1_sigma=1e-3):
+    """Seeded synthetic closed-shell inputs from the native generator (host arrays)."""
+    out = dict(eo=np.zeros(O), ev=np.zeros(V), t1=np.zeros((V, O)), t2=np.zeros((V, V, O, O)),
+               vvoo=np.zeros((V, V, O, O)), ovoo=np.zeros((O, V, O, O)),
+               vvvo=np.zeros((V, V, V, O)))
+    check(lib().xt_synth_closed_shell(O, V, seed, sigma, t1_sigma,
+                                      *[_dp(out[k]) for k in
+                                        ("eo", "ev", "t1", "t2", "vvoo", "ovoo", "vvvo")]))
+    return out
+
+
+def default_opts(**kw):
+    o = XtOpts()
+    lib().xt_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class CCSDTriples:
+    """One native context (one per host thread, one device)."""
+
+    def __init__(self, device: int = 0):
+        self._ctx = C.c_void_p()
+        check(lib().xt_create(device, C.byref(self._ctx)))
+        self.tiling = None
+
+    def close(self):
+        if self._ctx:
+            lib().xt_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tiling(self, tiling: Tiling, restricted: bool = True):
+        occ = np.ascontiguousarray(tiling.occ, dtype=np.int32)
+        vir = np.ascontiguousarray(tiling.vir, dtype=np.int32)
+        check(lib().xt_set_tiling(self._ctx, tiling.O, tiling.V, len(occ), _ip(occ), len(vir),
+                                  _ip(vir), int(restricted)))
+        self.tiling = tiling
+
+    def num_tasks(self):
+        n = C.c_long()
+        check(lib().xt_num_tasks(self._ctx, C.byref(n)))
+        return n.value
+
+    def task_list(self):
+        n = self.num_tasks()
+        t = np.zeros(6 * n, dtype=np.int64)
+        f = np.zeros(n)
+        check(lib().xt_task_list(self._ctx, t.ctypes.data_as(C.POINTER(C.c_long)), _dp(f)))
+        return t.reshape(n, 6), f
+
+    def flops(self):
+        n = self.num_tasks()
+        per = np.zeros(max(n, 1))
+        tot = C.c_double()
+        parts = np.zeros(3)
+        check(lib().xt_flops(self._ctx, _dp(per), C.byref(tot), _dp(parts)))
+        return tot.value, per[:n], parts
+
+    def upload(self, s):
+        arrs = [np.ascontiguousarray(s[k], dtype=np.float64)
+                for k in ("eo", "ev", "t1", "t2", "vvoo", "ovoo", "vvvo")]
+        check(lib().xt_upload_closed_shell(self._ctx, *[_dp(a) for a in arrs]))
+
+    def synth_upload(self, seed, sigma=0.05, t1_sigma=1e-3):
+        check(lib().xt_synth_upload(self._ctx, seed, sigma, t1_sigma))
+
+    def run(self, precision=PREC_FP64, schedule=SCHED_DRIVER, use_graph=0, record_task_time=0,
+            workgroups=0, partials=True):
+        n = self.num_tasks()
+        o = default_opts(precision=precision, schedule=schedule, use_graph=use_graph,
+                         record_task_time=record_task_time, workgroups=workgroups)
+        e = np.zeros(2)
+        tp = np.zeros(2 * max(n, 1)) if partials else None
+        tm = np.zeros(max(n, 1), dtype=np.float32) if record_task_time else None
+        check(lib().xt_run(self._ctx, C.byref(o), _dp(e), _dp(tp) if tp is not None else None,
+                           tm.ctypes.data_as(C.POINTER(C.c_float)) if tm is not None else None))
+        ms = C.c_double()
+        check(lib().xt_last_kernel_ms(self._ctx, C.byref(ms)))
+        return dict(e4=float(e[0]), e5=float(e[1]),
+                    task_partials=tp[:2 * n].reshape(n, 2) if tp is not None else None,
+                    task_ms=tm[:n] if tm is not None else None, kernel_ms=ms.value)
+
+
+def _run_tasks(self, which, precision=PREC_FP64, schedule=SCHED_DRIVER, record_task_time=0,
+               workgroups=0):
+    which = np.ascontiguousarray(which, dtype=np.int64)
+    n = len(which)
+    o = default_opts(precision=precision, schedule=schedule, record_task_time=record_task_time,
+                     workgroups=workgroups)
+    e = np.zeros(2)
+    tp = np.zeros(2 * n)
+    tm = np.zeros(n, dtype=np.float32)
+    check(lib().xt_run_tasks(self._ctx, C.byref(o), n, which.ctypes.data_as(C.POINTER(C.c_long)),
+                             _dp(e), _dp(tp), tm.ctypes.data_as(C.POINTER(C.c_float))))
+    ms = C.c_double()
+    check(lib().xt_last_kernel_ms(self._ctx, C.byref(ms)))
+    return dict(e4=float(e[0]), e5=float(e[1]), task_partials=tp.reshape(n, 2),
+                task_ms=tm if record_task_time else None, kernel_ms=ms.value)
+
+
+def _task_codes(self, task):
+    codes = (C.c_uint64 * 3)()
+    check(lib().xt_task_codes(self._ctx, task, codes))
+    return tuple(int(x) for x in codes)
+
+
+CCSDTriples.run_tasks = _run_tasks
+CCSDTriples.task_codes = _task_codes
+
+
+def fully_fused_ccsd_t_gpu(stream, num_blocks, dims, dev_bufs, meta, noab, nvab, max_dims, evl,
+                           partial, done_copy=None):
+    """Per-task drop-in for ``fully_fused_ccsd_t_gpu<double>``
+    (ccsd_t_all_fused_nontcCuda_Hip_Sycl.cpp:2800-2817).
+
+    dims: (h1,h2,h3,p4,p5,p6); dev_bufs: device pointers of
+    (d1_t2, d1_v2, d2_t2, d2_v2, s1_t1, s1_v2); meta: dict of int32 host arrays
+    d1_size, d1_exec, d2_size, d2_exec, s1_size, s1_exec; max_dims: dict with
+    d1_t2, d1_v2, d2_t2, d2_v2, s1_t1, s1_v2 (doubles per slot); evl: 6 device
+    pointers; partial: device pointer of 2*num_blocks doubles."""
+    m = {k: np.ascontiguousarray(meta[k], dtype=np.int32) for k in
+         ("d1_size", "d1_exec", "d2_size", "d2_exec", "s1_size", "s1_exec")}
+    return check(lib().xt_fully_fused_ccsd_t_gpu(
+        stream, num_blocks, *dims, *dev_bufs,
+        _ip(m["d1_size"]), _ip(m["d1_exec"]), _ip(m["d2_size"]), _ip(m["d2_exec"]),
+        _ip(m["s1_size"]), _ip(m["s1_exec"]),
+        noab, max_dims["d1_t2"], max_dims["d1_v2"], nvab, max_dims["d2_t2"], max_dims["d2_v2"],
+        max_dims["s1_t1"], max_dims["s1_v2"], *evl, partial, done_copy))

@@ -1,0 +1,185 @@
+/*
+ * exachem_t.h -- C ABI of the MI355X-native closed-shell CCSD(T) triples
+ * energy path (library: exachem_amd/_lib/libexachem_t.so).
+ *
+ * Two call surfaces:
+ *
+ *  1. Per-task launcher, a drop-in for ExaChem's
+ *       fully_fused_ccsd_t_gpu<double>(...)
+ *     (exachem/cc/ccsd_t/ccsd_t_all_fused_nontcCuda_Hip_Sycl.cpp:2800-2817,
+ *      declared at exachem/cc/ccsd_t/ccsd_t_all_fused.hpp:35-53).  It takes the
+ *     reference's packed device buffers (built by ccsd_t_data_{s1,d1,d2}_new)
+ *     and host size/exec arrays and writes 2*num_blocks partial energies that
+ *     the caller sums and scales by the task factor, exactly like
+ *     hostEnergyReduce (ccsd_t_all_fused.hpp:19-32).
+ *
+ *  2. Whole-problem path replacing the fused task loop
+ *       CCSD_T_Fused_Driver<T>::execute(...)
+ *     (exachem/cc/ccsd_t/ccsd_t_fused_driver.hpp:91-546): closed-shell spatial
+ *     inputs resident in HBM, the reference's task list and factors, all tasks
+ *     in one persistent launch, E[4]/E[5] out with a fixed-order FP64 sum.
+ *
+ * Every function returns XT_OK (0) or a negative XT_E* code and never exits;
+ * xt_last_error() gives the message of the last failure on the calling thread.
+ * No HIP or torch types appear in the signatures: streams and events are
+ * passed as void* (hipStream_t / hipEvent_t or NULL).
+ */
+#ifndef EXACHEM_T_H
+#define EXACHEM_T_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XT_OK 0
+#define XT_EINVAL (-1)    /* bad argument (null pointer, bad size, limit exceeded)   */
+#define XT_ENOMEM (-2)    /* device or host allocation failed                         */
+#define XT_EHIP (-3)      /* HIP runtime error                                        */
+#define XT_ESTATE (-4)    /* call out of order (e.g. run before upload)               */
+#define XT_ENODEV (-5)    /* no usable gfx950 device                                  */
+
+/* Precision of the contractions. */
+#define XT_PREC_FP64 0   /* FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 everywhere            */
+#define XT_PREC_MIXED 1  /* split BF16x3 products on MFMA, FP32 accumulation, FP64 energy */
+
+/* Task ordering of the device work queue. */
+#define XT_SCHED_DRIVER 0   /* reference driver order (ccsd_t_fused_driver.hpp:368-383)   */
+#define XT_SCHED_ANALYTIC 1 /* longest-first by the analytic flop model                    */
+#define XT_SCHED_LEARNED 2  /* longest-first by the learned cost model (falls back to 1)   */
+
+/* ---------------------------------------------------------------- errors */
+/* Message of the last failing call on this thread ("" if none). */
+const char* xt_last_error(void);
+/* Library version string. */
+const char* xt_version(void);
+/* 1 if a gfx950 device is visible, 0 otherwise (never initialises more than HIP). */
+int xt_device_available(void);
+
+/* ------------------------------------------------- 1. per-task launcher */
+/*
+ * Replaces fully_fused_ccsd_t_gpu<double> (nontc:2800-2817).  Argument order
+ * and meaning follow the reference; stream/done_copy are hipStream_t /
+ * hipEvent_t (done_copy may be NULL).  The host arrays host_d1_size[7*noab],
+ * host_d1_exec[9*noab], host_d2_size[7*nvab], host_d2_exec[9*nvab],
+ * host_s1_size[6], host_s1_exec[9] are consumed before the call returns, so
+ * the caller may overwrite them at once (done_copy is recorded on return for
+ * callers that wait on it).  partial_energies is device memory of
+ * 2*num_blocks doubles, num_blocks = prod(ceil(base_size_x/4)).
+ * Limits of the reference (MAX_NOAB=50, MAX_NVAB=140, nontc:56-57) are
+ * checked and reported as XT_EINVAL instead of overflowing.
+ */
+int xt_fully_fused_ccsd_t_gpu(void* stream, size_t num_blocks, size_t base_size_h1b,
+                              size_t base_size_h2b, size_t base_size_h3b, size_t base_size_p4b,
+                              size_t base_size_p5b, size_t base_size_p6b,
+                              const double* df_dev_d1_t2_all, const double* df_dev_d1_v2_all,
+                              const double* df_dev_d2_t2_all, const double* df_dev_d2_v2_all,
+                              const double* df_dev_s1_t1_all, const double* df_dev_s1_v2_all,
+                              const int* host_d1_size, const int* host_d1_exec,
+                              const int* host_d2_size, const int* host_d2_exec,
+                              const int* host_s1_size, const int* host_s1_exec, size_t size_noab,
+                              size_t size_max_dim_d1_t2, size_t size_max_dim_d1_v2,
+                              size_t size_nvab, size_t size_max_dim_d2_t2,
+                              size_t size_max_dim_d2_v2, size_t size_max_dim_s1_t1,
+                              size_t size_max_dim_s1_v2, const double* dev_evl_sorted_h1b,
+                              const double* dev_evl_sorted_h2b, const double* dev_evl_sorted_h3b,
+                              const double* dev_evl_sorted_p4b, const double* dev_evl_sorted_p5b,
+                              const double* dev_evl_sorted_p6b, double* partial_energies,
+                              void* done_copy);
+
+/* --------------------------------------------- 2. whole-problem path */
+typedef struct xt_ctx xt_ctx;
+
+/* One context per host thread; no global mutable state is shared.  device = -1
+ * creates a host-only context (tiling, task list, flop count; no HIP calls). */
+int xt_create(int device, xt_ctx** out);
+int xt_destroy(xt_ctx* ctx);
+
+/*
+ * Tiling of the spin-orbital space |occ_a|occ_b|virt_a|virt_b| as built by
+ * setupMOIS (exachem/cholesky/cholesky_2e.cpp:186-277): the beta tiles mirror
+ * the alpha tiles; k_spin is 1 for the first half of each space and 2 for the
+ * second (exachem/cc/ccsd_t/ccsd_t.cpp:245-249).  occ_tiles[n_occ_tiles] sum
+ * to O and vir_tiles[n_vir_tiles] sum to V.  Builds the task list of the fused
+ * driver (ccsd_t_fused_driver.hpp:368-395) with its spin filter and factors.
+ */
+int xt_set_tiling(xt_ctx* ctx, int O, int V, int n_occ_tiles, const int* occ_tiles,
+                  int n_vir_tiles, const int* vir_tiles, int is_restricted);
+
+/* Number of energy tasks after the spin filter. */
+int xt_num_tasks(xt_ctx* ctx, long* n_out);
+/* tasks[6*n] = (h1b,h2b,h3b,p4b,p5b,p6b) in driver order, factors[n]. */
+int xt_task_list(xt_ctx* ctx, long* tasks, double* factors);
+/* Analytic flop count of helper_calculate_num_ops (fused_common.hpp:131-261):
+ * per_task[n] (optional) and the total (exact integer held in a double up to 2^53,
+ * also returned split as s1/d1/d2 in parts[3], optional). */
+int xt_flops(xt_ctx* ctx, double* per_task, double* total, double* parts);
+
+/*
+ * Closed-shell spatial inputs (host memory, row-major FP64):
+ *   eps_occ[O], eps_vir[V]      orbital energies (diagonal of F)
+ *   t1[V][O]                    t_i^a
+ *   t2[V][V][O][O]              t_ij^ab  (t_ij^ab = t_ji^ba)
+ *   vvoo[V][V][O][O]            (ai|bj)  ((ai|bj) = (bj|ai))
+ *   ovoo[O][V][O][O]            (ia|jk)  (symmetric in j,k)
+ *   vvvo[V][V][V][O]            (ab|ci)  (symmetric in a,b)
+ * Copied to HBM and re-laid out there for the kernels.  Must follow
+ * xt_set_tiling with the same O, V.
+ */
+int xt_upload_closed_shell(xt_ctx* ctx, const double* eps_occ, const double* eps_vir,
+                           const double* t1, const double* t2, const double* vvoo,
+                           const double* ovoo, const double* vvvo);
+
+/* Seeded synthetic closed-shell inputs, generated on the host by a
+ * counter-based generator (independent of thread count) and uploaded as by
+ * xt_upload_closed_shell.  Distributions: eps_occ ~ U[-2,-0.3] sorted,
+ * eps_vir ~ U[0.1,3] sorted, vvoo/ovoo/vvvo ~ N(0, sigma^2) with exact
+ * permutational symmetry, t2 = vvoo/(e_i+e_j-e_a-e_b), t1 ~ N(0, t1_sigma^2). */
+int xt_synth_upload(xt_ctx* ctx, uint64_t seed, double sigma, double t1_sigma);
+/* Same generator, writing host arrays (any output may be NULL to skip it). */
+int xt_synth_closed_shell(int O, int V, uint64_t seed, double sigma, double t1_sigma,
+                          double* eps_occ, double* eps_vir, double* t1, double* t2, double* vvoo,
+                          double* ovoo, double* vvvo);
+
+typedef struct xt_opts {
+  int   precision;   /* XT_PREC_FP64 or XT_PREC_MIXED                                */
+  int   schedule;    /* XT_SCHED_*                                                   */
+  int   use_graph;   /* 1: capture the launch sequence in a hipGraph and replay it   */
+  int   record_task_time; /* 1: record per-task device timestamps                    */
+  int   workgroups;  /* persistent grid size (0 = library default)                   */
+  int   reserved[3];
+} xt_opts;
+
+/* Default options (FP64, driver order, graph on, timing off). */
+void xt_default_opts(xt_opts* o);
+
+/*
+ * Runs every task.  e45[0] = E[4] ([T] correction), e45[1] = E[5] ((T)
+ * correction, which includes E[4]), summed in task order in FP64 with the
+ * factors applied per task.  task_partials[2*n] (optional) receives each
+ * task's (E[4], E[5]) contribution, factor included; task_ms[n] (optional)
+ * receives per-task device time in ms when opts->record_task_time is set.
+ */
+int xt_run(xt_ctx* ctx, const xt_opts* opts, double* e45, double* task_partials, float* task_ms);
+
+/* Runs only the listed tasks (indices into the task list, no repeats).
+ * e45 = their summed (E[4], E[5]) in the listed order; task_partials[2*n] and
+ * task_ms[n] (optional) follow the listed order. */
+int xt_run_tasks(xt_ctx* ctx, const xt_opts* opts, long n, const long* which, double* e45,
+                 double* task_partials, float* task_ms);
+
+/* Slot codes of one task (introspection for tests and tools): codes[0] d1,
+ * codes[1] d2 (6 bits per output bipartition g = 3*hs+ps: enabled, negative,
+ * a1, a2, b1, b2), codes[2] s1 (4 bits per slot 3*pa+hi: enabled, negative,
+ * b1, b2). */
+int xt_task_codes(xt_ctx* ctx, long task, uint64_t* codes);
+
+/* Device time of the last xt_run's main kernel (ms), from HIP events. */
+int xt_last_kernel_ms(xt_ctx* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EXACHEM_T_H */

@@ -1,0 +1,265 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes front end of the CPU checker.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+may import this module.  The product package (exachem_amd) never does.
+
+What each function restates and how it is pinned:
+
+* ``flops`` / ``task_list`` -- ``CCSD_T_Fused_Driver::calculate_performance_ops``
+  (exachem/cc/ccsd_t/ccsd_t_fused_driver.hpp:548-638) and
+  ``helper_calculate_num_ops`` (exachem/cc/ccsd_t/fused_common.hpp:131-261).
+  Pinned exactly by the five ``total_num_ops`` values in the reference's CI
+  outputs (tests/golden/kat_total_num_ops.json).
+* ``task_exec`` -- the exec-slot assignment of the three gathers
+  (singles.hpp:55-177, doubles1.hpp:57-188, doubles2.hpp:58-191).  Pinned by the
+  same flop KATs (every enabled slot is counted) and by the energy checks below.
+* ``expand_closed_shell`` -- spin-orbital V2/T1/T2 from closed-shell spatial
+  inputs (exachem/cholesky/v2tensors.cpp:67-90, cc/ccsd/cd_ccsd_cs_ann.cpp:19-35,
+  932-976).  Pinned by exact antisymmetry checks and by agreement of the
+  spin-orbital energy with an independent spatial closed-shell formula.
+* ``run`` -- gathers + ``total_fused_ccsd_t_cpu`` (ccsd_t_all_fused_cpu.hpp:20-631).
+  No reference energy vector exists for synthetic inputs (the reference's CI
+  energies need SCF/CD/CCSD, which cannot be reproduced here).  Pinned by:
+  a dense tile-free brute force of the antisymmetrised s1/d1/d2 expressions,
+  tile invariance, restricted == unrestricted on closed-shell inputs,
+  T1 = 0 => E[5] = E[4], scaling laws, and the independent spatial formula
+  (tests/test_oracle_invariants.py).
+* ``task_packed`` -- the packed buffers handed to the GPU launcher
+  (ccsd_t_all_fused.hpp:144-213).  Pinned through ``run`` (same code path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libccsd_t_oracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        ip, dp, lp = C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_long)
+        L.oracle_num_tasks.restype = C.c_long
+        L.oracle_num_tasks.argtypes = [C.c_int, C.c_int, ip, ip, C.c_int]
+        L.oracle_task_list.restype = C.c_long
+        L.oracle_task_list.argtypes = [C.c_int, C.c_int, ip, ip, C.c_int, lp, dp]
+        L.oracle_flops.restype = C.c_longdouble
+        L.oracle_flops.argtypes = [C.c_int, C.c_int, ip, ip, C.c_int,
+                                   C.POINTER(C.c_longdouble), dp, dp]
+        L.oracle_task_exec.restype = None
+        L.oracle_task_exec.argtypes = [C.c_int, C.c_int, ip, ip, C.c_int, lp, ip, ip, ip]
+        L.oracle_expand_closed_shell.restype = None
+        L.oracle_expand_closed_shell.argtypes = [C.c_int, C.c_int] + [dp] * 10
+        L.oracle_run.restype = C.c_int
+        L.oracle_run.argtypes = [C.c_int, C.c_int, ip, ip, dp, C.c_int] + [dp] * 5 + [
+            C.c_long, lp, dp, dp]
+        L.oracle_run_spatial.restype = C.c_int
+        L.oracle_run_spatial.argtypes = [C.c_int, C.c_int, ip, ip, dp, C.c_int, C.c_int,
+                                         C.c_int] + [dp] * 5 + [C.c_long, lp, dp, dp]
+        L.oracle_buffer_sizes.restype = None
+        L.oracle_buffer_sizes.argtypes = [C.c_int, C.c_int, ip, ip, lp]
+        L.oracle_task_packed.restype = C.c_int
+        L.oracle_task_packed.argtypes = [C.c_int, C.c_int, ip, ip, C.c_int] + [dp] * 5 + [
+            lp] + [dp] * 6 + [ip, ip]
+        L.oracle_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _l(a):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_long))
+
+
+class Tiling:
+    """Spin-orbital tiling |occ_a|occ_b|virt_a|virt_b| with k_spin as in
+    exachem/cc/ccsd_t/ccsd_t.cpp:245-249 (first half of each space alpha)."""
+
+    def __init__(self, occ_a, occ_b, vir_a, vir_b):
+        self.occ_a, self.occ_b = list(occ_a), list(occ_b)
+        self.vir_a, self.vir_b = list(vir_a), list(vir_b)
+        self.tiles = np.array(self.occ_a + self.occ_b + self.vir_a + self.vir_b, dtype=np.int32)
+        self.noab = len(self.occ_a) + len(self.occ_b)
+        self.nvab = len(self.vir_a) + len(self.vir_b)
+        sp = ([1] * (self.noab // 2) + [2] * (self.noab - self.noab // 2) +
+              [1] * (self.nvab // 2) + [2] * (self.nvab - self.nvab // 2))
+        self.spins = np.array(sp, dtype=np.int32)
+
+    @staticmethod
+    def uniform(n, ts):
+        return [ts] * (n // ts) + ([n % ts] if n % ts else [])
+
+    @classmethod
+    def closed_shell(cls, O, V, occ_tiles, vir_tiles):
+        if isinstance(occ_tiles, int):
+            occ_tiles = cls.uniform(O, occ_tiles)
+        if isinstance(vir_tiles, int):
+            vir_tiles = cls.uniform(V, vir_tiles)
+        assert sum(occ_tiles) == O and sum(vir_tiles) == V
+        return cls(occ_tiles, occ_tiles, vir_tiles, vir_tiles)
+
+
+def task_list(t: Tiling, restricted=True):
+    L = lib()
+    tiles, tp = _i(t.tiles)
+    spins, sp = _i(t.spins)
+    n = L.oracle_num_tasks(t.noab, t.nvab, tp, sp, int(restricted))
+    tasks, tkp = _l(np.zeros(6 * n, dtype=np.int64))
+    fac, fp = _d(np.zeros(n))
+    L.oracle_task_list(t.noab, t.nvab, tp, sp, int(restricted), tkp, fp)
+    return tasks.reshape(n, 6), fac
+
+
+def flops(t: Tiling, restricted=True, per_task=False):
+    """Returns (total as int, (s1, d1, d2)[, per-task array, comm elems])."""
+    L = lib()
+    tiles, tp = _i(t.tiles)
+    spins, sp = _i(t.spins)
+    n = L.oracle_num_tasks(t.noab, t.nvab, tp, sp, int(restricted))
+    out3 = (C.c_longdouble * 3)()
+    pt, ptp = _d(np.zeros(max(n, 1)))
+    ce, cep = _d(np.zeros(max(n, 1)))
+    tot = L.oracle_flops(t.noab, t.nvab, tp, sp, int(restricted), out3, ptp, cep)
+    parts = tuple(int(out3[i]) for i in range(3))
+    total = sum(parts)
+    assert abs(float(tot) - total) <= 1e-6 * total + 1
+    if per_task:
+        return total, parts, pt[:n], ce[:n]
+    return total, parts
+
+
+def task_exec(t: Tiling, task, restricted=True):
+    L = lib()
+    tiles, tp = _i(t.tiles)
+    spins, sp = _i(t.spins)
+    tk, tkp = _l(np.asarray(task, dtype=np.int64))
+    s1, s1p = _i(np.zeros(9))
+    d1, d1p = _i(np.zeros(9 * t.noab))
+    d2, d2p = _i(np.zeros(9 * t.nvab))
+    L.oracle_task_exec(t.noab, t.nvab, tp, sp, int(restricted), tkp, s1p, d1p, d2p)
+    return s1, d1.reshape(t.noab, 9), d2.reshape(t.nvab, 9)
+
+
+def expand_closed_shell(O, V, t1s, t2s, vvoo, ovoo, vvvo):
+    """Dense spin-orbital tensors (t1, t2, v2ijab, v2ijka, v2iabc)."""
+    L = lib()
+    nO, nV = 2 * O, 2 * V
+    ins = [_d(x) for x in (t1s, t2s, vvoo, ovoo, vvvo)]
+    outs = [np.zeros(s) for s in ((nV, nO), (nV, nV, nO, nO), (nO, nO, nV, nV),
+                                  (nO, nO, nO, nV), (nO, nV, nV, nV))]
+    L.oracle_expand_closed_shell(O, V, *[p for _, p in ins],
+                                 *[o.ctypes.data_as(C.POINTER(C.c_double)) for o in outs])
+    return tuple(outs)
+
+
+def run(t: Tiling, evl, so, restricted=True, which=None):
+    """Fused (T) on the CPU.  so = (t1, t2, v2ijab, v2ijka, v2iabc) dense
+    spin-orbital arrays; evl = spin-orbital energies in tile order.
+    Returns (E4, E5, per_task[n,2])."""
+    L = lib()
+    tiles, tp = _i(t.tiles)
+    spins, sp = _i(t.spins)
+    ev, evp = _d(evl)
+    arrs = [_d(x) for x in so]
+    if which is None:
+        n = L.oracle_num_tasks(t.noab, t.nvab, tp, sp, int(restricted))
+        wh, whp = _l(np.zeros(1, dtype=np.int64))
+        nw = 0
+    else:
+        wh, whp = _l(np.asarray(which, dtype=np.int64))
+        n = nw = len(wh)
+    e, ep = _d(np.zeros(2))
+    pt, ptp = _d(np.zeros(2 * max(n, 1)))
+    rc = L.oracle_run(t.noab, t.nvab, tp, sp, evp, int(restricted), *[p for _, p in arrs],
+                      nw, whp, ep, ptp)
+    if rc != 0:
+        raise RuntimeError("oracle_run failed")
+    return e[0], e[1], pt[:2 * n].reshape(n, 2)
+
+
+def run_spatial(t: Tiling, evl, s, restricted=True, which=None):
+    """Fused (T) on the CPU with spin-orbital elements formed on the fly from
+    closed-shell spatial inputs s (dict with t1, t2, vvoo, ovoo, vvvo).
+    Returns (E4, E5, per_task[n,2])."""
+    L = lib()
+    tiles, tp = _i(t.tiles)
+    spins, sp = _i(t.spins)
+    ev, evp = _d(evl)
+    O, V = s["t1"].shape[1], s["t1"].shape[0]
+    arrs = [_d(s[k]) for k in ("t1", "t2", "vvoo", "ovoo", "vvvo")]
+    if which is None:
+        n = L.oracle_num_tasks(t.noab, t.nvab, tp, sp, int(restricted))
+        wh, whp = _l(np.zeros(1, dtype=np.int64))
+        nw = 0
+    else:
+        wh, whp = _l(np.asarray(which, dtype=np.int64))
+        n = nw = len(wh)
+    e, ep = _d(np.zeros(2))
+    pt, ptp = _d(np.zeros(2 * max(n, 1)))
+    rc = L.oracle_run_spatial(t.noab, t.nvab, tp, sp, evp, int(restricted), O, V,
+                              *[p for _, p in arrs], nw, whp, ep, ptp)
+    if rc != 0:
+        raise RuntimeError(f"oracle_run_spatial failed ({rc})")
+    return e[0], e[1], pt[:2 * n].reshape(n, 2)
+
+
+def buffer_sizes(t: Tiling):
+    L = lib()
+    tiles, tp = _i(t.tiles)
+    spins, sp = _i(t.spins)
+    out, op = _l(np.zeros(8, dtype=np.int64))
+    L.oracle_buffer_sizes(t.noab, t.nvab, tp, sp, op)
+    keys = ["s1_t1", "s1_v2", "d1_t2", "d1_v2", "d2_t2", "d2_v2", "max_d1", "max_d2"]
+    return dict(zip(keys, (int(x) for x in out)))
+
+
+def task_packed(t: Tiling, so, task, restricted=True):
+    """Packed buffers + metadata of one task as the reference's gathers make them."""
+    L = lib()
+    tiles, tp = _i(t.tiles)
+    spins, sp = _i(t.spins)
+    arrs = [_d(x) for x in so]
+    sz = buffer_sizes(t)
+    bufs = {k: np.zeros(sz[k]) for k in ["s1_t1", "s1_v2", "d1_t2", "d1_v2", "d2_t2", "d2_v2"]}
+    nmeta = 6 + 9 + 16 * t.noab + 16 * t.nvab
+    meta, mp = _i(np.zeros(nmeta))
+    counts, cp = _i(np.zeros(3))
+    tk, tkp = _l(np.asarray(task, dtype=np.int64))
+    L.oracle_task_packed(t.noab, t.nvab, tp, sp, int(restricted), *[p for _, p in arrs], tkp,
+                         *[bufs[k].ctypes.data_as(C.POINTER(C.c_double)) for k in
+                           ["s1_t1", "s1_v2", "d1_t2", "d1_v2", "d2_t2", "d2_v2"]], mp, cp)
+    o = 0
+    m = {}
+    for name, cnt in [("s1_size", 6), ("s1_exec", 9), ("d1_size", 7 * t.noab),
+                      ("d1_exec", 9 * t.noab), ("d2_size", 7 * t.nvab), ("d2_exec", 9 * t.nvab)]:
+        m[name] = meta[o:o + cnt].copy()
+        o += cnt
+    return bufs, m, tuple(int(c) for c in counts), sz
+
+
+def num_threads():
+    return lib().oracle_num_threads()

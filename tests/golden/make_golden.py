@@ -1,0 +1,102 @@
+"""Writes the golden fixtures of the GPU parity tests (run here, on the CPU).
+
+Inputs come from the product's seeded generator (xt_synth_closed_shell), so
+the GPU tests regenerate the identical arrays on the GPU box; the fixture
+stores checksums of every input array to prove that.  Expected energies come
+from the CPU oracle (oracle/ccsd_t_oracle.cpp, the restatement of ExaChem's
+CPU fused (T) path).
+
+    python tests/golden/make_golden.py small      # config 0 + small uneven cases (~1 min)
+    python tests/golden/make_golden.py c1sample   # sampled tasks of O=40,V=300 (long)
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import exachem_amd as X  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+SEED = 20241202
+SIGMA, T1_SIGMA = 0.05, 1e-3
+
+
+def checksums(s):
+    return {k: [float(np.sum(s[k])), float(np.sum(s[k] * s[k]))] for k in sorted(s)}
+
+
+def oracle_case(name, O_, V_, occ, vir, seed, restricted=True, which=None):
+    s = X.synth_closed_shell(O_, V_, seed, SIGMA, T1_SIGMA)
+    t = O.Tiling(occ, occ, vir, vir)
+    evl = np.concatenate([s["eo"], s["eo"], s["ev"], s["ev"]])
+    t0 = time.time()
+    e4, e5, pt = O.run_spatial(t, evl, s, restricted, which)
+    dt = time.time() - t0
+    tasks, fac = O.task_list(t, restricted)
+    tot, parts, per_task, _ = O.flops(t, restricted, per_task=True)
+    print(f"{name}: E4={e4!r} E5={e5!r} ({dt:.1f}s, {O.num_threads()} threads)", flush=True)
+    out = dict(name=name, O=O_, V=V_, occ_tiles=occ, vir_tiles=vir, seed=seed, sigma=SIGMA,
+               t1_sigma=T1_SIGMA, restricted=restricted, checksums=checksums(s),
+               n_tasks=int(len(fac)), total_num_ops=int(tot), oracle_seconds=dt,
+               oracle_threads=O.num_threads())
+    if which is None:
+        out.update(E4=e4, E5=e5, task_partials=pt.tolist())
+    else:
+        out.update(which=[int(w) for w in which], task_partials=pt.tolist(),
+                   task_tuples=tasks[np.asarray(which)].tolist(),
+                   task_flops=[float(per_task[w]) for w in which])
+    return out
+
+
+def small():
+    cases = [
+        oracle_case("config0", 8, 48, [8], [16, 16, 16], SEED),
+        # uneven, non-multiple-of-4 tiles; restricted and unrestricted
+        oracle_case("uneven_small", 7, 22, [3, 4], [5, 9, 8], SEED + 1),
+        oracle_case("uneven_small_unrestricted", 7, 22, [3, 4], [5, 9, 8], SEED + 1,
+                    restricted=False),
+        oracle_case("one_tile", 5, 13, [5], [13], SEED + 2),
+    ]
+    with open(os.path.join(HERE, "parity_small.json"), "w") as f:
+        json.dump(cases, f, indent=1)
+
+
+def c1sample():
+    occ, vir = X.triples.uniform_tiles(40, 16), X.triples.uniform_tiles(300, 32)
+    t = O.Tiling(occ, occ, vir, vir)
+    tasks, fac = O.task_list(t)
+    tot, parts, per_task, _ = O.flops(t, True, per_task=True)
+    rng = np.random.default_rng(7)
+    pick = {int(np.argmax(per_task)), int(np.argmin(per_task)), 0, len(fac) - 1}
+    # tasks with repeated tiles (factor 1/6 and 1/2 paths) and a mixed-spin task
+    for k in range(len(fac)):
+        h1, h2, h3, p4, p5, p6 = tasks[k]
+        if h1 == h2 == h3 and p4 == p5 == p6:
+            pick.add(k)
+            break
+    pick |= set(int(x) for x in rng.choice(len(fac), 4, replace=False))
+    which = sorted(pick)
+    # one task at a time so progress is visible; the oracle's per-task values do
+    # not depend on which other tasks run in the same call
+    parts_ = []
+    for w in which:
+        c = oracle_case("config1_sample", 40, 300, occ, vir, SEED, which=[w])
+        parts_.append(c)
+    case = parts_[0]
+    case["which"] = which
+    case["task_partials"] = [p["task_partials"][0] for p in parts_]
+    case["task_tuples"] = [p["task_tuples"][0] for p in parts_]
+    case["task_flops"] = [p["task_flops"][0] for p in parts_]
+    case["oracle_seconds"] = sum(p["oracle_seconds"] for p in parts_)
+    with open(os.path.join(HERE, "parity_c1_sample.json"), "w") as f:
+        json.dump(case, f, indent=1)
+
+
+if __name__ == "__main__":
+    {"small": small, "c1sample": c1sample}[sys.argv[1]]()

@@ -1,0 +1,130 @@
+"""GPU parity of the HIP path against the CPU oracle (golden fixtures written
+by tests/golden/make_golden.py from the oracle on identical seeded inputs).
+
+Tolerance (FP64): |dE| <= 1e-10 Eh + 1e-13 |E| for E[4], E[5] and each task's
+contribution.  The synthetic inputs give |E| of 1..1e3 Eh at these sizes, so
+the relative term only matters where an absolute 1e-10 Eh would ask for more
+than FP64 rounding of a ~1e8-term sum can give."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+pytestmark = pytest.mark.gpu
+
+ABS, REL = 1e-10, 1e-13
+
+
+def close(a, b):
+    return abs(a - b) <= ABS + REL * abs(b)
+
+
+def small_cases():
+    return json.load(open(os.path.join(HERE, "golden", "parity_small.json")))
+
+
+def load_case(xt, case):
+    s = xt.synth_closed_shell(case["O"], case["V"], case["seed"], case["sigma"], case["t1_sigma"])
+    for k, (a, b) in case["checksums"].items():
+        assert float(np.sum(s[k])) == a and float(np.sum(s[k] * s[k])) == b, k
+    ctx = xt.CCSDTriples(0)
+    ctx.set_tiling(xt.Tiling(case["O"], case["V"], case["occ_tiles"], case["vir_tiles"]),
+                   case["restricted"])
+    ctx.upload(s)
+    return ctx, s
+
+
+@pytest.mark.parametrize("name", ["one_tile", "uneven_small", "uneven_small_unrestricted",
+                                  "config0"])
+def test_fp64_matches_oracle(xt, name):
+    case = next(c for c in small_cases() if c["name"] == name)
+    print(f"[gpu] case {name} O={case['O']} V={case['V']}", flush=True)
+    ctx, _ = load_case(xt, case)
+    r = ctx.run()
+    assert close(r["e4"], case["E4"]), (r["e4"], case["E4"])
+    assert close(r["e5"], case["E5"]), (r["e5"], case["E5"])
+    ref = np.asarray(case["task_partials"])
+    got = r["task_partials"]
+    bad = [(k, got[k].tolist(), ref[k].tolist()) for k in range(len(ref))
+           if not (close(got[k, 0], ref[k, 0]) and close(got[k, 1], ref[k, 1]))]
+    assert not bad, bad[:5]
+    ctx.close()
+
+
+def test_schedule_and_grid_do_not_change_bits(xt):
+    case = next(c for c in small_cases() if c["name"] == "uneven_small")
+    ctx, _ = load_case(xt, case)
+    a = ctx.run(schedule=xt.SCHED_DRIVER)
+    b = ctx.run(schedule=xt.SCHED_ANALYTIC, workgroups=7)
+    c = ctx.run(schedule=xt.SCHED_DRIVER, workgroups=1)
+    assert a["e4"] == b["e4"] == c["e4"] and a["e5"] == b["e5"] == c["e5"]
+    assert np.array_equal(a["task_partials"], b["task_partials"])
+    # subset runs reproduce the full run's per-task values bit for bit
+    n = ctx.num_tasks()
+    which = [n - 1, 0, n // 2]
+    s = ctx.run_tasks(which, schedule=xt.SCHED_ANALYTIC)
+    assert np.array_equal(s["task_partials"], a["task_partials"][which])
+    ctx.close()
+
+
+def test_packed_dropin_matches_oracle(xt, oracle):
+    """xt_fully_fused_ccsd_t_gpu fed with the reference's packed buffers
+    (oracle's restatement of ccsd_t_data_{s1,d1,d2}_new) gives each task's
+    energy after the host-side reduction of hostEnergyReduce."""
+    from tests.hiprt import DevBuf, Stream
+    st = Stream()
+    for name in ("uneven_small", "config0"):
+        case = next(c for c in small_cases() if c["name"] == name)
+        print(f"[gpu] packed drop-in {name}", flush=True)
+        s = xt.synth_closed_shell(case["O"], case["V"], case["seed"], case["sigma"],
+                                  case["t1_sigma"])
+        occ, vir = case["occ_tiles"], case["vir_tiles"]
+        t = oracle.Tiling(occ, occ, vir, vir)
+        so = oracle.expand_closed_shell(case["O"], case["V"], s["t1"], s["t2"], s["vvoo"],
+                                        s["ovoo"], s["vvvo"])
+        tasks, fac = oracle.task_list(t, case["restricted"])
+        evl = np.concatenate([s["eo"], s["eo"], s["ev"], s["ev"]])
+        offs = np.concatenate([[0], np.cumsum(t.tiles)])
+        ref = np.asarray(case["task_partials"])
+        ks = range(len(tasks)) if len(tasks) <= 40 else list(range(0, len(tasks), 3))
+        for k in ks:
+            bufs, meta, counts, sz = oracle.task_packed(t, so, tasks[k], case["restricted"])
+            names = ("d1_t2", "d1_v2", "d2_t2", "d2_v2", "s1_t1", "s1_v2")
+            dev = {n: DevBuf.from_array(bufs[n]) for n in names}
+            dims = [int(t.tiles[b]) for b in tasks[k]]
+            ev = [DevBuf.from_array(evl[offs[b]:offs[b] + t.tiles[b]]) for b in tasks[k]]
+            nb = int(np.prod([(d + 3) // 4 for d in dims]))
+            part = DevBuf(16 * nb)
+            max_dims = {n: sz[n] // (9 if n.startswith("s1") else
+                                     (sz["max_d1"] if n.startswith("d1") else sz["max_d2"]))
+                        for n in names}
+            xt.fully_fused_ccsd_t_gpu(st.h, nb, dims, [dev[n].ptr for n in names], meta, t.noab,
+                                      t.nvab, max_dims, [e.ptr for e in ev], part.ptr)
+            st.sync()
+            p = part.to_array(2 * nb)
+            e4 = fac[k] * np.sum(p[:nb])
+            e5 = fac[k] * np.sum(p[nb:])
+            for b_ in list(dev.values()) + ev + [part]:
+                b_.free()
+            assert close(e4, ref[k, 0]) and close(e5, ref[k, 1]), (name, k, e4, ref[k, 0])
+    st.destroy()
+
+
+def test_config1_sampled_tasks(xt):
+    path = os.path.join(HERE, "golden", "parity_c1_sample.json")
+    if not os.path.exists(path):
+        pytest.skip("config-1 sample fixture not generated")
+    case = json.load(open(path))
+    print(f"[gpu] config1 sample tasks {case['which']}", flush=True)
+    ctx = xt.CCSDTriples(0)
+    ctx.set_tiling(xt.Tiling(case["O"], case["V"], case["occ_tiles"], case["vir_tiles"]))
+    ctx.synth_upload(case["seed"], case["sigma"], case["t1_sigma"])
+    r = ctx.run_tasks(case["which"])
+    ref = np.asarray(case["task_partials"])
+    for i, k in enumerate(case["which"]):
+        assert close(r["task_partials"][i, 0], ref[i, 0]), (k, r["task_partials"][i], ref[i])
+        assert close(r["task_partials"][i, 1], ref[i, 1]), (k, r["task_partials"][i], ref[i])
+    ctx.close()

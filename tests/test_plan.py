@@ -1,0 +1,159 @@
+"""CPU tests of the product's host side: the C ABI loads and exports what the
+header declares, the planner reproduces the reference driver's task list,
+factors, slot enables and flop counts (checked against the oracle), the
+generator keeps its promised symmetries, and errors come back as codes."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_library_exports_every_header_symbol(xt):
+    syms = xt.header_symbols()
+    assert len(syms) >= 15
+    L = xt.lib()
+    for s in syms:
+        assert hasattr(L, s), s
+
+
+def host_ctx(xt, O_, V_, occ, vir, restricted=True):
+    ctx = xt.CCSDTriples(-1)
+    ctx.set_tiling(xt.Tiling(O_, V_, list(occ), list(vir)), restricted)
+    return ctx
+
+
+CASES = [
+    (9, 36, [9], [36]),                         # seh2 tiling
+    (21, 14, [21], [14]),                       # butanol2 tiling
+    (29, 103, [29], [40, 40, 23]),              # uracil tiling
+    (8, 48, [8], [16, 16, 16]),                 # config 0
+    (7, 22, [3, 4], [5, 9, 8]),                 # uneven
+    (13, 40, [6, 7], [16, 13, 11]),             # uneven, config-3-like ranges
+]
+
+
+@pytest.mark.parametrize("O_,V_,occ,vir", CASES)
+@pytest.mark.parametrize("restricted", [True, False])
+def test_task_list_factors_flops_match_oracle(xt, oracle, O_, V_, occ, vir, restricted):
+    ctx = host_ctx(xt, O_, V_, occ, vir, restricted)
+    tasks, fac = ctx.task_list()
+    t = oracle.Tiling(occ, occ, vir, vir)
+    otasks, ofac = oracle.task_list(t, restricted)
+    assert np.array_equal(tasks, otasks)
+    assert np.array_equal(fac, ofac)
+    tot, per, parts = ctx.flops()
+    ototal, oparts, oper, _ = oracle.flops(t, restricted, per_task=True)
+    assert tot == ototal and np.array_equal(per, oper)
+    assert tuple(int(p) for p in parts) == oparts
+
+
+def test_uracil_known_answer_through_product(xt):
+    ctx = host_ctx(xt, 29, 103, [29], [40, 40, 23])
+    assert ctx.flops()[0] == 62789556886348
+
+
+def _decode(codes):
+    d1 = [(codes[0] >> (6 * g)) & 63 for g in range(9)]
+    d2 = [(codes[1] >> (6 * g)) & 63 for g in range(9)]
+    s1 = [(codes[2] >> (4 * s)) & 15 for s in range(9)]
+    return d1, d2, s1
+
+
+@pytest.mark.parametrize("O_,V_,occ,vir", CASES[3:])
+@pytest.mark.parametrize("restricted", [True, False])
+def test_slot_enables_match_reference_exec_flags(xt, oracle, O_, V_, occ, vir, restricted):
+    """Bipartition g = 3*hs + ps is fed by reference d1 slot 3*((ps+1)%3)+hs
+    and d2 slot 3*ps+(hs+1)%3; s1 slot numbering is shared (3*pa+hi).  The
+    product's enable bit must equal 'some h7 (p7) tile has exec >= 0' and the
+    contraction must span every tile of one spin."""
+    ctx = host_ctx(xt, O_, V_, occ, vir, restricted)
+    t = oracle.Tiling(occ, occ, vir, vir)
+    tasks, _ = ctx.task_list()
+    spins = np.asarray(t.spins)
+    for k in range(len(tasks)):
+        d1, d2, s1 = _decode(ctx.task_codes(k))
+        os1, od1, od2 = oracle.task_exec(t, tasks[k], restricted)
+        for g in range(9):
+            hs, ps = divmod(g, 3)
+            r1 = 3 * ((ps + 1) % 3) + hs
+            r2 = 3 * ps + (hs + 1) % 3
+            on1 = od1[:, r1] >= 0
+            on2 = od2[:, r2] >= 0
+            assert bool(d1[g] & 1) == bool(on1.any()), (k, g)
+            assert bool(d2[g] & 1) == bool(on2.any()), (k, g)
+            if on1.any():  # all occupied tiles of exactly one spin
+                sp = set(spins[:t.noab][on1])
+                assert len(sp) == 1 and on1.sum() == np.sum(spins[:t.noab] == sp.pop())
+            if on2.any():
+                sp = set(spins[t.noab:][on2])
+                assert len(sp) == 1 and on2.sum() == np.sum(spins[t.noab:] == sp.pop())
+        for sl in range(9):
+            assert bool(s1[sl] & 1) == bool(os1[sl] >= 0), (k, sl)
+
+
+def test_generator_symmetry_and_determinism(xt):
+    a = xt.synth_closed_shell(5, 11, 42)
+    b = xt.synth_closed_shell(5, 11, 42)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["vvoo"], a["vvoo"].transpose(1, 0, 3, 2))
+    assert np.array_equal(a["ovoo"], a["ovoo"].transpose(0, 1, 3, 2))
+    assert np.array_equal(a["vvvo"], a["vvvo"].transpose(1, 0, 2, 3))
+    assert np.array_equal(a["t2"], a["t2"].transpose(1, 0, 3, 2))
+    D = (a["eo"][None, None, :, None] + a["eo"][None, None, None, :]) - (
+        a["ev"][:, None, None, None] + a["ev"][None, :, None, None])
+    assert np.allclose(a["t2"] * D, a["vvoo"], rtol=1e-15, atol=0)
+    assert np.all(np.diff(a["eo"]) >= 0) and np.all(np.diff(a["ev"]) >= 0)
+    assert a["eo"].min() >= -2.0 and a["eo"].max() <= -0.3
+    assert a["ev"].min() >= 0.1 and a["ev"].max() <= 3.0
+    c = xt.synth_closed_shell(5, 11, 43)
+    assert not np.array_equal(a["vvvo"], c["vvvo"])
+    big = xt.synth_closed_shell(6, 40, 1)
+    assert abs(big["vvvo"].std() - 0.05) < 0.002 and abs(big["vvvo"].mean()) < 0.002
+
+
+def test_generator_matches_golden_checksums(xt):
+    cases = json.load(open(os.path.join(HERE, "golden", "parity_small.json")))
+    for case in cases:
+        s = xt.synth_closed_shell(case["O"], case["V"], case["seed"], case["sigma"],
+                                  case["t1_sigma"])
+        for k, (a, b) in case["checksums"].items():
+            assert float(np.sum(s[k])) == a and float(np.sum(s[k] * s[k])) == b, (case["name"], k)
+
+
+def test_error_codes_not_exits(xt):
+    L = xt.lib()
+    ctx = xt.CCSDTriples(-1)
+    occ = np.array([3, 0], dtype=np.int32)
+    vir = np.array([5], dtype=np.int32)
+    rc = L.xt_set_tiling(ctx._ctx, 3, 5, 2, occ.ctypes.data_as(C.POINTER(C.c_int)), 1,
+                         vir.ctypes.data_as(C.POINTER(C.c_int)), 1)
+    assert rc == xt.XT_EINVAL and b"empty" in L.xt_last_error()
+    occ = np.array([3], dtype=np.int32)
+    rc = L.xt_set_tiling(ctx._ctx, 4, 5, 1, occ.ctypes.data_as(C.POINTER(C.c_int)), 1,
+                         vir.ctypes.data_as(C.POINTER(C.c_int)), 1)
+    assert rc == xt.XT_EINVAL
+    with pytest.raises(xt.XtError) as ei:
+        ctx.num_tasks()
+    assert ei.value.code == xt.XT_ESTATE
+    ctx.set_tiling(xt.Tiling(3, 5, [3], [5]))
+    with pytest.raises(xt.XtError) as ei:
+        ctx.run()
+    assert ei.value.code == xt.XT_ENODEV
+    # per-task drop-in: limits of the reference are errors, not overflows
+    z = np.zeros(7 * 60, dtype=np.int32)
+    zp = z.ctypes.data_as(C.POINTER(C.c_int))
+    one = C.c_void_p(8)
+    rc = L.xt_fully_fused_ccsd_t_gpu(None, 1, 4, 4, 4, 4, 4, 4, *[one] * 6, *[zp] * 6,
+                                     51, 1, 1, 1, 1, 1, 1, 1, *[one] * 6, one, None)
+    assert rc == xt.XT_EINVAL and b"noab" in L.xt_last_error()
+    rc = L.xt_fully_fused_ccsd_t_gpu(None, 2, 4, 4, 4, 4, 4, 4, *[one] * 6, *[zp] * 6,
+                                     2, 1, 1, 2, 1, 1, 1, 1, *[one] * 6, one, None)
+    assert rc == xt.XT_EINVAL and b"num_blocks" in L.xt_last_error()
+    rc = L.xt_fully_fused_ccsd_t_gpu(None, 1, 4, 4, 4, 4, 4, 4, *[None] * 6, *[zp] * 6,
+                                     2, 1, 1, 2, 1, 1, 1, 1, *[one] * 6, one, None)
+    assert rc == xt.XT_EINVAL
