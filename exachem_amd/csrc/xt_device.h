@@ -10,13 +10,13 @@ namespace xt {
 // (ccsd_t_fused_driver.hpp:368-395) as the kernels see it.  Positions 0..5 are
 // h1,h2,h3,p4,p5,p6.
 struct TaskDev {
-  int      dim[6];      // tile extents
-  int      off[6];      // first spatial orbital of each tile (occ or virt range)
-  int      nsub[6];     // ceil(dim/4): 4-wide sub-tiles per position
-  int      n_sub;       // product of nsub
-  int      unit_begin;  // first work unit of this task
-  int      n_units;     // work units of this task
-  int      subs_per_unit;
+  int       dim[6];      // tile extents
+  int       off[6];      // first spatial orbital of each tile (occ or virt range)
+  int       nsub[6];     // ceil(dim/4): 4-wide sub-tiles per position
+  int       nbox[6];     // ceil(nsub/2): 2^6-sub-tile boxes per position
+  int       n_sub;       // product of nsub
+  int       pad_;
+  long long n_tickets;   // 64 * prod(nbox): queue tickets of this task (some empty)
   // Per output bipartition g = 3*hs + ps (hs: single hole position 0..2,
   // ps: single particle position 0..2), 6 bits each:
   //   bit0 enabled, bit1 negative sign, bit2 a1, bit3 a2, bit4 b1, bit5 b2
@@ -29,37 +29,63 @@ struct TaskDev {
   double   flops;       // analytic flop count of the task
 };
 
-// HBM-resident closed-shell inputs in kernel layouts.
+// HBM-resident closed-shell inputs in kernel layouts.  The contraction index
+// (last) of every operand array is padded with zeros to Op = roundup(O, 8)
+// or Vp = roundup(V, 8), so K-chunks load with aligned vector loads.  The *a
+// arrays hold the antisymmetrised difference used by same-spin blocks, formed
+// once in FP64 exactly as the spin-orbital element would be (x - y).
 struct SpatialDev {
-  int           O, V;
+  int           O, V, Op, Vp;
   const double* eo;    // [O]
   const double* ev;    // [V]
-  const double* t1;    // [V][O]          t_i^a
-  const double* t2h;   // [V][V][O][O]    t_ij^ab            (j contiguous; d1 operand)
-  const double* t2p;   // [O][O][V][V]    t2p[i][j][a][b] = t_ij^ab (b contiguous; d2 operand)
-  const double* ovk;   // [O][V][O][O]    (ia|jk)            (k contiguous; d1 operand)
-  const double* xv;    // [V][V][O][V]    xv[c][b][i][a] = (ac|bi) (a contiguous; d2 operand)
-  const double* vvoo;  // [V][V][O][O]    (ai|bj)            (s1 operand)
+  const double* t1;    // [V][O]           t_i^a
+  const double* t2h;   // [V][V][O][Op]    t2h[a][b][i][k] = t_ik^ab            (d1 M side)
+  const double* t2ha;  // [V][V][O][Op]    t2h[a][b][i][k] - t2h[b][a][i][k]
+  const double* ovk;   // [O][V][O][Op]    ovk[i][a][j][k] = (ia|jk)            (d1 N side)
+  const double* ovka;  // [O][V][O][Op]    ovk[i][a][j][k] - ovk[j][a][i][k]
+  const double* xv;    // [V][V][O][Vp]    xv[c][b][i][a] = (ac|bi)             (d2 M side)
+  const double* xva;   // [V][V][O][Vp]    xv[c][b][i][a] - xv[b][c][i][a]
+  const double* t2p;   // [O][O][V][Vp]    t2p[i][j][a][b] = t_ij^ab            (d2 N side)
+  const double* t2pa;  // [O][O][V][Vp]    t2p[i][j][a][b] - t2p[j][i][a][b]
+  const double* vvoo;  // [V][V][O][O]     (ai|bj)                              (s1)
 };
 
+// Device copies of the inputs in the layouts above.
+struct SpatialBufs {
+  double *eo, *ev, *t1, *t2h, *t2ha, *ovk, *ovka, *xv, *xva, *t2p, *t2pa, *vvoo;
+};
+
+// Work queues: one per XCD.  Queue q holds the tasks qtask[qbeg[q]..qbeg[q+1])
+// in processing order; entry e owns tickets [qtick[e], qtick[e] + n_tickets)
+// of its queue.  A ticket is one 4^6 sub-tile; 64 consecutive tickets form a
+// 2^6 box of neighbouring sub-tiles so that the workgroups resident on one XCD
+// share operand panels in its L2.
+constexpr int kQueues = 8;
+
 struct RunDev {
-  const TaskDev* tasks;
-  const int*     unit_task;   // [n_units] task of each unit
-  const int*     order;       // [n_units] queue order (unit ids)
-  int            n_units;
-  int            n_tasks;
-  double*        unit_e;      // [2*n_units] per-unit (E4,E5) partials, no factor
-  unsigned int*  queue;       // device work counter
-  unsigned long long* task_t0;  // optional per-task first/last timestamps
-  unsigned long long* task_t1;
+  const TaskDev*      tasks;
+  const int*          qtask;   // [n_entries]
+  const long long*    qtick;   // [n_entries]
+  const int*          qbeg;    // [kQueues+1]
+  const long long*    qtotal;  // [kQueues]
+  unsigned long long* qctr;    // [kQueues] ticket counters (zeroed before a run)
+  unsigned long long* acc;     // [4*n_tasks] (E4 lo, E4 hi, E5 lo, E5 hi): 128-bit
+                               // two's-complement fixed point, scale 2^64 per Eh
+  unsigned long long* cost;    // [n_tasks] summed workgroup wall-clock ticks (optional)
+  int                 n_tasks;
 };
 
 // Kernel launchers (defined in xt_kernels.hip).  Return hipError_t as int.
-int launch_relayout(const double* t2_in, const double* ovoo_in, const double* vvvo_in, int O,
-                    int V, double* t2h, double* t2p, double* ovk, double* xv, void* stream);
+// Builds the padded / antisymmetrised operand arrays from the caller's
+// device copies t2 [V][V][O][O], ovoo [O][V][O][O] and vvvo [V][V][V][O]
+// (consumed; vvvo is freed as soon as xv exists to bound peak memory).
+// Returns 0 or a hipError_t / -1 for allocation failure.
+int build_operands(int O, int V, double* t2, double* ovoo, double*& vvvo, SpatialBufs& B,
+                   void* stream);
 int launch_spatial_fp64(const SpatialDev& S, const RunDev& R, int grid, void* stream);
-int launch_reduce(const TaskDev* tasks, int n_tasks, const double* unit_e, double* task_e,
-                  double* e45, void* stream);
+int launch_reduce(const TaskDev* tasks, int n_tasks, const unsigned long long* acc,
+                  double* task_e, double* e45, void* stream);
+int launch_zero_u64(unsigned long long* p, long n, void* stream);
 
 struct PackedArgs {
   int           dim[6];

@@ -125,8 +125,6 @@ int synth(int O, int V, uint64_t seed, double sigma, double t1s, double* eo, dou
 }
 
 // ------------------------------------------------------------- planning
-constexpr int kSubsPerUnit = 32;
-
 inline void pair_of(int s, int& a, int& b) {
   a = (s == 0) ? 1 : 0;
   b = (s == 2) ? 1 : 2;
@@ -141,7 +139,6 @@ struct Plan {
   std::vector<long>       tasks;  // 6 per task, tile ids
   std::vector<double>     factors;
   std::vector<xt::TaskDev> dev;
-  long                    n_units = 0;
 };
 
 // Slot logic of one task, derived from the tile spins.  It restates the
@@ -253,7 +250,6 @@ int build_plan(Plan& P, int O, int V, const std::vector<int>& occ, const std::ve
             }
   const long nt = (long) P.factors.size();
   P.dev.resize(nt);
-  long unit = 0;
   for(long k = 0; k < nt; k++) {
     xt::TaskDev& T = P.dev[k];
     std::memset(&T, 0, sizeof(T));
@@ -269,11 +265,12 @@ int build_plan(Plan& P, int O, int V, const std::vector<int>& occ, const std::ve
     }
     int nd1, nd2, ns1;
     slot_codes(sp, T.d1code, T.d2code, T.s1code, nd1, nd2, ns1);
-    T.factor        = P.factors[k];
-    T.subs_per_unit = kSubsPerUnit;
-    T.n_units       = (T.n_sub + kSubsPerUnit - 1) / kSubsPerUnit;
-    T.unit_begin    = (int) unit;
-    unit += T.n_units;
+    T.factor    = P.factors[k];
+    T.n_tickets = 64;
+    for(int p = 0; p < 6; p++) {
+      T.nbox[p] = (T.nsub[p] + 1) / 2;
+      T.n_tickets *= T.nbox[p];
+    }
     double prod = 1;
     for(int p = 0; p < 6; p++) prod *= T.dim[p];
     // helper_calculate_num_ops (fused_common.hpp:131-261): 2*prod per s1 slot,
@@ -281,8 +278,6 @@ int build_plan(Plan& P, int O, int V, const std::vector<int>& occ, const std::ve
     // spin halves have O (V) orbitals each.
     T.flops = 2.0 * prod * (ns1 + (double) nd1 * O + (double) nd2 * V);
   }
-  if(unit > (long) 0x7fffffff) return fail(XT_EINVAL, "too many work units");
-  P.n_units = unit;
   return XT_OK;
 }
 
@@ -301,21 +296,27 @@ struct xt_ctx {
   int         n_cu   = 0;
   Plan        plan;
   bool        have_plan = false, have_inputs = false;
-  // device inputs
-  double *eo = nullptr, *ev = nullptr, *t1 = nullptr, *t2h = nullptr, *t2p = nullptr,
-         *ovk = nullptr, *xv = nullptr, *vvoo = nullptr;
-  // device plan
-  xt::TaskDev* d_tasks = nullptr;
-  int *d_unit_task = nullptr, *d_order = nullptr;
-  double *d_unit_e = nullptr, *d_task_e = nullptr, *d_e45 = nullptr;
-  unsigned int*       d_queue   = nullptr;
-  unsigned long long *d_t0 = nullptr, *d_t1 = nullptr;
+  // device inputs (kernel layouts)
+  xt::SpatialBufs in = {};
+  // device plan and run state
+  xt::TaskDev*        d_tasks  = nullptr;
+  int*                d_qtask  = nullptr;
+  int*                d_qbeg   = nullptr;
+  long long*          d_qtick  = nullptr;
+  long long*          d_qtotal = nullptr;
+  unsigned long long* d_qctr   = nullptr;
+  unsigned long long* d_acc    = nullptr;
+  unsigned long long* d_cost   = nullptr;
+  double *            d_task_e = nullptr, *d_e45 = nullptr;
   long                dev_tasks_n = -1;
   hipEvent_t          ev0 = nullptr, ev1 = nullptr;
   double              last_ms = 0.0;
+  std::vector<double> learned_cost;  // per-task predicted cost (learned schedule)
 
   void free_inputs() {
-    for(double** p: {&eo, &ev, &t1, &t2h, &t2p, &ovk, &xv, &vvoo})
+    double** ps[] = {&in.eo,  &in.ev,  &in.t1,  &in.t2h, &in.t2ha, &in.ovk,
+                     &in.ovka, &in.xv, &in.xva, &in.t2p, &in.t2pa, &in.vvoo};
+    for(double** p: ps)
       if(*p) {
         (void) hipFree(*p);
         *p = nullptr;
@@ -323,21 +324,19 @@ struct xt_ctx {
     have_inputs = false;
   }
   void free_plan_dev() {
-    if(d_tasks) (void) hipFree(d_tasks);
-    if(d_unit_task) (void) hipFree(d_unit_task);
-    if(d_order) (void) hipFree(d_order);
-    if(d_unit_e) (void) hipFree(d_unit_e);
-    if(d_task_e) (void) hipFree(d_task_e);
-    if(d_e45) (void) hipFree(d_e45);
-    if(d_queue) (void) hipFree(d_queue);
-    if(d_t0) (void) hipFree(d_t0);
-    if(d_t1) (void) hipFree(d_t1);
-    d_tasks = nullptr;
-    d_unit_task = d_order = nullptr;
-    d_unit_e = d_task_e = d_e45 = nullptr;
-    d_queue = nullptr;
-    d_t0 = d_t1 = nullptr;
-    dev_tasks_n = -1;
+    void* ps[] = {d_tasks, d_qtask, d_qbeg, d_qtick, d_qtotal, d_qctr, d_acc, d_cost, d_task_e, d_e45};
+    for(void* p: ps)
+      if(p) (void) hipFree(p);
+    d_tasks  = nullptr;
+    d_qtask  = nullptr;
+    d_qbeg   = nullptr;
+    d_qtick  = nullptr;
+    d_qtotal = nullptr;
+    d_qctr   = nullptr;
+    d_acc    = nullptr;
+    d_cost   = nullptr;
+    d_task_e = d_e45 = nullptr;
+    dev_tasks_n      = -1;
   }
 };
 
@@ -476,40 +475,39 @@ static int upload_common(xt_ctx* c, const double* eo, const double* ev, const do
   const size_t o = c->plan.O, v = c->plan.V;
   XT_HIP(hipSetDevice(c->device));
   c->free_inputs();
-  auto alloc = [&](double** p, size_t n) -> int {
-    hipError_t e = hipMalloc((void**) p, n * sizeof(double));
+  hipStream_t st = c->stream;
+  double *    t2_d = nullptr, *ovoo_d = nullptr, *vvvo_d = nullptr;
+  auto        cleanup = [&]() {
+    for(double* p: {t2_d, ovoo_d, vvvo_d})
+      if(p) (void) hipFree(p);
+    c->free_inputs();
+  };
+  auto up = [&](double** dst, const double* src, size_t n) -> int {
+    hipError_t e = hipMalloc((void**) dst, n * sizeof(double));
     if(e != hipSuccess) {
-      *p = nullptr;
+      *dst = nullptr;
       return fail(XT_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(double)) + " bytes failed");
     }
+    e = hipMemcpyAsync(*dst, src, n * sizeof(double), hipMemcpyHostToDevice, st);
+    if(e != hipSuccess) return fail(XT_EHIP, std::string("upload copy: ") + hipGetErrorString(e));
     return XT_OK;
   };
   int rc;
-  if((rc = alloc(&c->eo, o)) || (rc = alloc(&c->ev, v)) || (rc = alloc(&c->t1, v * o)) ||
-     (rc = alloc(&c->t2h, v * v * o * o)) || (rc = alloc(&c->t2p, v * v * o * o)) ||
-     (rc = alloc(&c->ovk, o * v * o * o)) || (rc = alloc(&c->xv, v * v * v * o)) ||
-     (rc = alloc(&c->vvoo, v * v * o * o))) {
-    c->free_inputs();
+  if((rc = up(&c->in.eo, eo, o)) || (rc = up(&c->in.ev, ev, v)) || (rc = up(&c->in.t1, t1, v * o)) ||
+     (rc = up(&c->in.vvoo, vvoo, v * v * o * o)) || (rc = up(&t2_d, t2, v * v * o * o)) ||
+     (rc = up(&ovoo_d, ovoo, o * v * o * o)) || (rc = up(&vvvo_d, vvvo, v * v * v * o))) {
+    cleanup();
     return rc;
   }
-  hipStream_t st = c->stream;
-  XT_HIP(hipMemcpyAsync(c->eo, eo, o * sizeof(double), hipMemcpyHostToDevice, st));
-  XT_HIP(hipMemcpyAsync(c->ev, ev, v * sizeof(double), hipMemcpyHostToDevice, st));
-  XT_HIP(hipMemcpyAsync(c->t1, t1, v * o * sizeof(double), hipMemcpyHostToDevice, st));
-  XT_HIP(hipMemcpyAsync(c->vvoo, vvoo, v * v * o * o * sizeof(double), hipMemcpyHostToDevice, st));
-  // staging copies of t2 / ovoo / vvvo: t2 lands in t2h directly; ovoo in ovk;
-  // vvvo needs a scratch buffer for the relayout.
-  double* vvvo_d = nullptr;
-  if((rc = alloc(&vvvo_d, v * v * v * o))) return rc;
-  XT_HIP(hipMemcpyAsync(c->t2h, t2, v * v * o * o * sizeof(double), hipMemcpyHostToDevice, st));
-  XT_HIP(hipMemcpyAsync(c->ovk, ovoo, o * v * o * o * sizeof(double), hipMemcpyHostToDevice, st));
-  XT_HIP(hipMemcpyAsync(vvvo_d, vvvo, v * v * v * o * sizeof(double), hipMemcpyHostToDevice, st));
-  int err = xt::launch_relayout(c->t2h, c->ovk, vvvo_d, (int) o, (int) v, c->t2h, c->t2p, c->ovk,
-                                c->xv, st);
-  hipError_t se = hipStreamSynchronize(st);
-  (void) hipFree(vvvo_d);
-  if(err) return fail(XT_EHIP, std::string("relayout: ") + hipGetErrorString((hipError_t) err));
-  if(se != hipSuccess) return fail(XT_EHIP, std::string("upload: ") + hipGetErrorString(se));
+  int err = xt::build_operands((int) o, (int) v, t2_d, ovoo_d, vvvo_d, c->in, st);
+  if(err) {
+    cleanup();
+    return fail(err == (int) hipErrorOutOfMemory ? XT_ENOMEM : XT_EHIP,
+                std::string("operand layout: ") + hipGetErrorString((hipError_t) err));
+  }
+  (void) hipFree(t2_d);
+  (void) hipFree(ovoo_d);
+  if(vvvo_d) (void) hipFree(vvvo_d);
   c->have_inputs = true;
   return XT_OK;
 }
@@ -549,100 +547,146 @@ void xt_default_opts(xt_opts* o) {
 }
 
 static int ensure_plan_dev(xt_ctx* c) {
-  const Plan& P = c->plan;
-  const long  nt = (long) P.dev.size();
-  if(c->dev_tasks_n == nt) return XT_OK;
+  const Plan& P  = c->plan;
+  const long  nt = std::max<long>((long) P.dev.size(), 1);
+  if(c->dev_tasks_n == (long) P.dev.size()) return XT_OK;
   c->free_plan_dev();
-  std::vector<int> unit_task(std::max<long>(P.n_units, 1));
-  for(long k = 0; k < nt; k++)
-    for(int u = 0; u < P.dev[k].n_units; u++) unit_task[P.dev[k].unit_begin + u] = (int) k;
-  XT_HIP(hipMalloc((void**) &c->d_tasks, sizeof(xt::TaskDev) * std::max<long>(nt, 1)));
-  XT_HIP(hipMalloc((void**) &c->d_unit_task, sizeof(int) * unit_task.size()));
-  XT_HIP(hipMalloc((void**) &c->d_order, sizeof(int) * unit_task.size()));
-  XT_HIP(hipMalloc((void**) &c->d_unit_e, sizeof(double) * 2 * unit_task.size()));
-  XT_HIP(hipMalloc((void**) &c->d_task_e, sizeof(double) * 2 * std::max<long>(nt, 1)));
+  XT_HIP(hipMalloc((void**) &c->d_tasks, sizeof(xt::TaskDev) * nt));
+  XT_HIP(hipMalloc((void**) &c->d_qtask, sizeof(int) * nt));
+  XT_HIP(hipMalloc((void**) &c->d_qtick, sizeof(long long) * nt));
+  XT_HIP(hipMalloc((void**) &c->d_qbeg, sizeof(int) * (xt::kQueues + 1)));
+  XT_HIP(hipMalloc((void**) &c->d_qtotal, sizeof(long long) * xt::kQueues));
+  XT_HIP(hipMalloc((void**) &c->d_qctr, sizeof(unsigned long long) * xt::kQueues));
+  XT_HIP(hipMalloc((void**) &c->d_acc, sizeof(unsigned long long) * 4 * nt));
+  XT_HIP(hipMalloc((void**) &c->d_cost, sizeof(unsigned long long) * nt));
+  XT_HIP(hipMalloc((void**) &c->d_task_e, sizeof(double) * 2 * nt));
   XT_HIP(hipMalloc((void**) &c->d_e45, sizeof(double) * 2));
-  XT_HIP(hipMalloc((void**) &c->d_queue, sizeof(unsigned int) * 4));
-  XT_HIP(hipMalloc((void**) &c->d_t0, sizeof(unsigned long long) * std::max<long>(nt, 1)));
-  XT_HIP(hipMalloc((void**) &c->d_t1, sizeof(unsigned long long) * std::max<long>(nt, 1)));
-  XT_HIP(hipMemcpyAsync(c->d_tasks, P.dev.data(), sizeof(xt::TaskDev) * nt, hipMemcpyHostToDevice,
-                        c->stream));
-  XT_HIP(hipMemcpyAsync(c->d_unit_task, unit_task.data(), sizeof(int) * unit_task.size(),
-                        hipMemcpyHostToDevice, c->stream));
-  XT_HIP(hipStreamSynchronize(c->stream));
-  c->dev_tasks_n = nt;
+  if(!P.dev.empty())
+    XT_HIP(hipMemcpy(c->d_tasks, P.dev.data(), sizeof(xt::TaskDev) * P.dev.size(),
+                     hipMemcpyHostToDevice));
+  c->dev_tasks_n = (long) P.dev.size();
   return XT_OK;
 }
 
-static std::vector<int> unit_order(const Plan& P, int schedule) {
-  const long       nt = (long) P.dev.size();
-  std::vector<int> task_order(nt);
-  std::iota(task_order.begin(), task_order.end(), 0);
-  if(schedule != XT_SCHED_DRIVER)
-    std::stable_sort(task_order.begin(), task_order.end(),
-                     [&](int a, int b) { return P.dev[a].flops > P.dev[b].flops; });
-  std::vector<int> order;
-  order.reserve(P.n_units);
-  for(int t: task_order)
-    for(int u = 0; u < P.dev[t].n_units; u++) order.push_back(P.dev[t].unit_begin + u);
-  return order;
+// Queues of one run.  Every task stays in one queue (its sub-tiles share
+// operand panels in that XCD's L2).  Driver schedule: tasks dealt round-robin
+// in driver order.  Cost schedules: longest-processing-time first onto the
+// least-loaded queue, each queue processed in decreasing cost.
+struct Queues {
+  std::vector<int>       task, beg;
+  std::vector<long long> tick, total;
+};
+
+static Queues build_queues(const Plan& P, const std::vector<int>& sel, int schedule,
+                           const std::vector<double>* cost) {
+  const int                     nq = xt::kQueues;
+  std::vector<std::vector<int>> per(nq);
+  if(schedule == XT_SCHED_DRIVER) {
+    for(size_t i = 0; i < sel.size(); i++) per[i % nq].push_back(sel[i]);
+  }
+  else {
+    auto c = [&](int t) { return cost ? (*cost)[t] : P.dev[t].flops; };
+    std::vector<int> order = sel;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return c(a) > c(b); });
+    std::vector<double> load(nq, 0.0);
+    for(int t: order) {
+      const int q = (int) (std::min_element(load.begin(), load.end()) - load.begin());
+      load[q] += c(t);
+      per[q].push_back(t);
+    }
+  }
+  Queues Q;
+  Q.beg.push_back(0);
+  for(int q = 0; q < nq; q++) {
+    long long tk = 0;
+    for(int t: per[q]) {
+      Q.task.push_back(t);
+      Q.tick.push_back(tk);
+      tk += P.dev[t].n_tickets;
+    }
+    Q.beg.push_back((int) Q.task.size());
+    Q.total.push_back(tk);
+  }
+  return Q;
 }
 
-// Launches the persistent kernel over the units of `tasks_sel` (all tasks if
-// null) in the order given by the schedule, then the fixed-order reductions.
+// Launches the persistent kernel over the tasks `sel` (all if null), then the
+// per-task fixed-point -> FP64 conversion and the fixed-order sum.
 static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_sel) {
-  const Plan& P  = c->plan;
-  const long  nt = (long) P.dev.size();
-  std::vector<int> order;
-  if(tasks_sel) {
-    std::vector<int> sel = *tasks_sel;
-    if(o.schedule != XT_SCHED_DRIVER)
-      std::stable_sort(sel.begin(), sel.end(),
-                       [&](int a, int b) { return P.dev[a].flops > P.dev[b].flops; });
-    for(int t: sel)
-      for(int u = 0; u < P.dev[t].n_units; u++) order.push_back(P.dev[t].unit_begin + u);
+  const Plan&      P  = c->plan;
+  const long       nt = (long) P.dev.size();
+  std::vector<int> sel;
+  if(tasks_sel) sel = *tasks_sel;
+  else {
+    sel.resize(nt);
+    std::iota(sel.begin(), sel.end(), 0);
   }
-  else order = unit_order(P, o.schedule);
+  const std::vector<double>* learned =
+    (o.schedule == XT_SCHED_LEARNED && (long) c->learned_cost.size() == nt) ? &c->learned_cost
+                                                                            : nullptr;
+  Queues      Q  = build_queues(P, sel, o.schedule, learned);
   hipStream_t st = c->stream;
-  if(!order.empty())
-    XT_HIP(hipMemcpyAsync(c->d_order, order.data(), sizeof(int) * order.size(),
+  if(!Q.task.empty()) {
+    XT_HIP(hipMemcpyAsync(c->d_qtask, Q.task.data(), sizeof(int) * Q.task.size(),
                           hipMemcpyHostToDevice, st));
-  XT_HIP(hipMemsetAsync(c->d_queue, 0, sizeof(unsigned int) * 4, st));
-  XT_HIP(hipMemsetAsync(c->d_unit_e, 0, sizeof(double) * 2 * std::max<long>(P.n_units, 1), st));
-  const bool timing = o.record_task_time != 0;
-  if(timing) {
-    XT_HIP(hipMemsetAsync(c->d_t0, 0xff, sizeof(unsigned long long) * nt, st));
-    XT_HIP(hipMemsetAsync(c->d_t1, 0, sizeof(unsigned long long) * nt, st));
+    XT_HIP(hipMemcpyAsync(c->d_qtick, Q.tick.data(), sizeof(long long) * Q.tick.size(),
+                          hipMemcpyHostToDevice, st));
   }
+  XT_HIP(hipMemcpyAsync(c->d_qbeg, Q.beg.data(), sizeof(int) * Q.beg.size(), hipMemcpyHostToDevice,
+                        st));
+  XT_HIP(hipMemcpyAsync(c->d_qtotal, Q.total.data(), sizeof(long long) * Q.total.size(),
+                        hipMemcpyHostToDevice, st));
+  int err = xt::launch_zero_u64(c->d_qctr, xt::kQueues, st);
+  if(!err) err = xt::launch_zero_u64(c->d_acc, 4 * std::max<long>(nt, 1), st);
+  if(!err) err = xt::launch_zero_u64(c->d_cost, std::max<long>(nt, 1), st);
+  if(err) return fail(XT_EHIP, std::string("zero: ") + hipGetErrorString((hipError_t) err));
   xt::SpatialDev S;
   S.O    = P.O;
   S.V    = P.V;
-  S.eo   = c->eo;
-  S.ev   = c->ev;
-  S.t1   = c->t1;
-  S.t2h  = c->t2h;
-  S.t2p  = c->t2p;
-  S.ovk  = c->ovk;
-  S.xv   = c->xv;
-  S.vvoo = c->vvoo;
+  S.Op   = (P.O + 7) / 8 * 8;
+  S.Vp   = (P.V + 7) / 8 * 8;
+  S.eo   = c->in.eo;
+  S.ev   = c->in.ev;
+  S.t1   = c->in.t1;
+  S.t2h  = c->in.t2h;
+  S.t2ha = c->in.t2ha;
+  S.ovk  = c->in.ovk;
+  S.ovka = c->in.ovka;
+  S.xv   = c->in.xv;
+  S.xva  = c->in.xva;
+  S.t2p  = c->in.t2p;
+  S.t2pa = c->in.t2pa;
+  S.vvoo = c->in.vvoo;
   xt::RunDev R;
-  R.tasks     = c->d_tasks;
-  R.unit_task = c->d_unit_task;
-  R.order     = c->d_order;
-  R.n_units   = (int) order.size();
-  R.n_tasks   = (int) nt;
-  R.unit_e    = c->d_unit_e;
-  R.queue     = c->d_queue;
-  R.task_t0   = timing ? c->d_t0 : nullptr;
-  R.task_t1   = timing ? c->d_t1 : nullptr;
-  int grid    = o.workgroups > 0 ? o.workgroups : c->n_cu * 2;
-  grid        = (int) std::min<long>(grid, std::max<long>((long) order.size(), 1));
+  R.tasks   = c->d_tasks;
+  R.qtask   = c->d_qtask;
+  R.qtick   = c->d_qtick;
+  R.qbeg    = c->d_qbeg;
+  R.qtotal  = c->d_qtotal;
+  R.qctr    = c->d_qctr;
+  R.acc     = c->d_acc;
+  R.cost    = o.record_task_time ? c->d_cost : nullptr;
+  R.n_tasks = (int) nt;
+  long long tickets = 0;
+  for(long long t: Q.total) tickets += t;
+  int grid = o.workgroups > 0 ? o.workgroups : c->n_cu * 2;
+  grid     = (int) std::max<long long>(1, std::min<long long>(grid, tickets));
   XT_HIP(hipEventRecord(c->ev0, st));
-  int err = xt::launch_spatial_fp64(S, R, grid, st);
+  err = xt::launch_spatial_fp64(S, R, grid, st);
   if(err) return fail(XT_EHIP, std::string("launch: ") + hipGetErrorString((hipError_t) err));
   XT_HIP(hipEventRecord(c->ev1, st));
-  err = xt::launch_reduce(c->d_tasks, (int) nt, c->d_unit_e, c->d_task_e, c->d_e45, st);
+  err = xt::launch_reduce(c->d_tasks, (int) nt, c->d_acc, c->d_task_e, c->d_e45, st);
   if(err) return fail(XT_EHIP, std::string("reduce: ") + hipGetErrorString((hipError_t) err));
+  return XT_OK;
+}
+
+// Summed workgroup time of each task (ms), from wall-clock ticks.
+static int task_times(xt_ctx* c, long nt, float* task_ms) {
+  std::vector<unsigned long long> ticks(nt);
+  XT_HIP(hipMemcpy(ticks.data(), c->d_cost, nt * 8, hipMemcpyDeviceToHost));
+  int rate_khz = 100000;
+  (void) hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device);
+  for(long k = 0; k < nt; k++) task_ms[k] = (float) ((double) ticks[k] / rate_khz);
   return XT_OK;
 }
 
@@ -655,17 +699,6 @@ static int check_run_state(xt_ctx* c, const xt_opts& o) {
     return fail(XT_EINVAL, "xt_run: bad schedule");
   XT_HIP(hipSetDevice(c->device));
   return ensure_plan_dev(c);
-}
-
-static int task_times(xt_ctx* c, long nt, float* task_ms) {
-  std::vector<unsigned long long> t0(nt), t1(nt);
-  XT_HIP(hipMemcpy(t0.data(), c->d_t0, nt * 8, hipMemcpyDeviceToHost));
-  XT_HIP(hipMemcpy(t1.data(), c->d_t1, nt * 8, hipMemcpyDeviceToHost));
-  int rate_khz = 100000;
-  (void) hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device);
-  for(long k = 0; k < nt; k++)
-    task_ms[k] = t1[k] > t0[k] ? (float) ((double) (t1[k] - t0[k]) / rate_khz) : 0.f;
-  return XT_OK;
 }
 
 int xt_run(xt_ctx* c, const xt_opts* opts, double* e45, double* task_partials, float* task_ms) {

@@ -13,8 +13,8 @@
 //     so the two run back to back into one 64x64 FP64 accumulator held in
 //     MFMA registers (v_mfma_f64_16x16x4f64, 2x2 tiles per wave) and the
 //     result is added once into the LDS t3 tile;
-//   * operands are staged through LDS in K-chunks of 16 with a register
-//     prefetch of the next chunk;
+//   * operands are staged through LDS in K-chunks of 32 (aligned 16-byte
+//     loads from zero-padded arrays) with a register prefetch of the next chunk;
 //   * s1 (ccsd_t_all_fused_cpu.hpp:472-578) and the denominator-weighted
 //     energy (586-619) run on the LDS tile; the 256 partials are reduced
 //     with wave64 shuffles and a fixed-order cross-wave sum.
@@ -27,17 +27,19 @@ namespace xt {
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int NT      = 256;       // threads per workgroup
-constexpr int KC      = 16;        // K chunk staged per LDS round
+constexpr int KC      = 32;        // K chunk staged per LDS round
 constexpr int LDK     = KC + 1;    // padded row stride (doubles) of staged operands
 constexpr int T3N     = 4096;      // 4^6 sub-tile
 constexpr int S1_T1   = 9 * 16;    // s1 T1 blocks in LDS
-constexpr int STAGE_D = S1_T1 + 9 * 256;  // staging union, >= 2*64*LDK
+constexpr int STAGE_D = 2 * 64 * LDK;  // staging union: A,B chunks; s1 blocks (9*(16+256))
+static_assert(STAGE_D >= S1_T1 + 9 * 256, "staging too small for s1 blocks");
 
 struct __align__(16) Smem {
   double t3[T3N];
   double stage[STAGE_D];
   double eps[24];
   double red[8];
+  int    sub[6], dim[6], off[6];  // current sub-tile: position, extents, offsets
 };
 
 __device__ __forceinline__ int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -169,30 +171,35 @@ __device__ __forceinline__ void subtile_energy(Smem& sm, const int* sub, const i
 // tensors (spin-orbital antisymmetrised values of <pq||rs> and T2, exactly the
 // numbers the reference's gathers would copy out of TAMM blocks).
 // ---------------------------------------------------------------------------
-struct RowPtrs {
-  const double* p1;
-  const double* p2;
-  double        c1, c2;
+// One staged operand row: pointer to its contraction run and the coefficient
+// (+-1 or 0) that folds the equation sign and the spin-orbital selection.
+struct RowPtr {
+  const double* p;
+  double        c;
 };
 
-__device__ __forceinline__ void fetch4(const RowPtrs& rp, int k0, int K, double* x, double* y) {
+__device__ __forceinline__ void fetch8(const RowPtr& rp, int k0, int Kp, double* x) {
+  if(rp.c != 0.0 && k0 < Kp) {
+    const double2* q = reinterpret_cast<const double2*>(rp.p + k0);
+    double2        v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3];
+    x[0] = v0.x; x[1] = v0.y; x[2] = v1.x; x[3] = v1.y;
+    x[4] = v2.x; x[5] = v2.y; x[6] = v3.x; x[7] = v3.y;
+  }
+  else {
 #pragma unroll
-  for(int j = 0; j < 4; j++) {
-    const int k = k0 + j;
-    x[j]        = (rp.c1 != 0.0 && k < K) ? rp.p1[k] : 0.0;
-    y[j]        = (rp.c2 != 0.0 && k < K) ? rp.p2[k] : 0.0;
+    for(int j = 0; j < 8; j++) x[j] = 0.0;
   }
 }
 
-__device__ void spatial_subtile(Smem& sm, const SpatialDev& S, const TaskDev& T, const int* sub,
-                                double& e4, double& e5) {
-  const int tid = threadIdx.x;
-  const int O = S.O, V = S.V;
-  __syncthreads();  // previous sub-tile's energy phase is done with LDS
+__device__ void spatial_subtile(Smem& sm, const SpatialDev& S, uint64_t d1code, uint64_t d2code,
+                                uint64_t s1code, double& e4, double& e5) {
+  const int  tid = threadIdx.x;
+  const int  O = S.O, V = S.V;
+  const int* sub = sm.sub;
   for(int e = tid; e < T3N; e += NT) sm.t3[e] = 0.0;
   if(tid < 24) {
     const int p = tid >> 2, i = tid & 3, g = 4 * sub[p] + i;
-    sm.eps[tid] = g < T.dim[p] ? (p < 3 ? S.eo : S.ev)[T.off[p] + g] : 0.0;
+    sm.eps[tid] = g < sm.dim[p] ? (p < 3 ? S.eo : S.ev)[sm.off[p] + g] : 0.0;
   }
   __syncthreads();
 
@@ -202,8 +209,8 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, const TaskDev& T,
 
 #pragma unroll 1
   for(int g = 0; g < 9; g++) {
-    const unsigned c1 = (unsigned) (T.d1code >> (6 * g)) & 63u;
-    const unsigned c2 = (unsigned) (T.d2code >> (6 * g)) & 63u;
+    const unsigned c1 = (unsigned) (d1code >> (6 * g)) & 63u;
+    const unsigned c2 = (unsigned) (d2code >> (6 * g)) & 63u;
     const int      n1 = (c1 & 1) ? cdiv(O, KC) : 0;
     const int      n2 = (c2 & 1) ? cdiv(V, KC) : 0;
     if(n1 + n2 == 0) continue;
@@ -216,34 +223,41 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, const TaskDev& T,
               gPB = 4 * sub[3 + pB] + (row >> 4);
     const int gHA = 4 * sub[hA] + (row & 3), gHB = 4 * sub[hB] + ((row >> 2) & 3),
               gPS = 4 * sub[3 + ps] + (row >> 4);
-    const bool mok = gHS < T.dim[hs] && gPA < T.dim[3 + pA] && gPB < T.dim[3 + pB];
-    const bool nok = gHA < T.dim[hA] && gHB < T.dim[hB] && gPS < T.dim[3 + ps];
-    const size_t HS = T.off[hs] + (mok ? gHS : 0), PA = T.off[3 + pA] + (mok ? gPA : 0),
-                 PB = T.off[3 + pB] + (mok ? gPB : 0);
-    const size_t HA = T.off[hA] + (nok ? gHA : 0), HB = T.off[hB] + (nok ? gHB : 0),
-                 PS = T.off[3 + ps] + (nok ? gPS : 0);
-    RowPtrs a1r, b1r, a2r, b2r;
+    const bool mok = gHS < sm.dim[hs] && gPA < sm.dim[3 + pA] && gPB < sm.dim[3 + pB];
+    const bool nok = gHA < sm.dim[hA] && gHB < sm.dim[hB] && gPS < sm.dim[3 + ps];
+    const size_t HS = sm.off[hs] + (mok ? gHS : 0), PA = sm.off[3 + pA] + (mok ? gPA : 0),
+                 PB = sm.off[3 + pB] + (mok ? gPB : 0);
+    const size_t HA = sm.off[hA] + (nok ? gHA : 0), HB = sm.off[hB] + (nok ? gHB : 0),
+                 PS = sm.off[3 + ps] + (nok ? gPS : 0);
+    // operand rows c1*x - c2*y of the spin-orbital elements: both terms ->
+    // the antisymmetrised array; one term -> that array (sign folded into c)
+    auto pick = [](bool ok, unsigned code, unsigned bx, unsigned by, double sgn, const double* both,
+                   const double* x, const double* y) {
+      RowPtr r;
+      const bool hx = (code & bx) != 0, hy = (code & by) != 0;
+      r.p = hx && hy ? both : (hx ? x : y);
+      r.c = !ok || !(hx || hy) ? 0.0 : (hx ? sgn : -sgn);
+      return r;
+    };
+    const size_t Op = S.Op, Vp = S.Vp;
+    RowPtr       a1r, b1r, a2r, b2r;
     {
-      const double s = (c1 & 2) ? -1.0 : 1.0;
-      a1r.p1 = S.t2h + ((PA * V + PB) * O + HS) * O;  // t(pA,pB,hs,h7)
-      a1r.p2 = S.t2h + ((PB * V + PA) * O + HS) * O;  // t(pB,pA,hs,h7) = t(pA,pB,h7,hs)
-      a1r.c1 = (mok && (c1 & 4)) ? s : 0.0;
-      a1r.c2 = (mok && (c1 & 8)) ? -s : 0.0;
-      b1r.p1 = S.ovk + ((HB * V + PS) * O + HA) * O;  // (hA h7|hB ps)
-      b1r.p2 = S.ovk + ((HA * V + PS) * O + HB) * O;  // (hB h7|hA ps)
-      b1r.c1 = (nok && (c1 & 16)) ? 1.0 : 0.0;
-      b1r.c2 = (nok && (c1 & 32)) ? -1.0 : 0.0;
+      const double sg = (c1 & 2) ? -1.0 : 1.0;
+      const size_t xo = ((PA * V + PB) * O + HS) * Op, yo = ((PB * V + PA) * O + HS) * Op;
+      // t(pA,pB,hs,h7) - t(pA,pB,h7,hs) = t2h[pA,pB,hs,:] - t2h[pB,pA,hs,:]
+      a1r = pick(mok, c1, 4, 8, sg, S.t2ha + xo, S.t2h + xo, S.t2h + yo);
+      // (hA h7|hB ps) - (hB h7|hA ps) = ovk[hB,ps,hA,:] - ovk[hA,ps,hB,:]
+      const size_t bxo = ((HB * V + PS) * O + HA) * Op, byo = ((HA * V + PS) * O + HB) * Op;
+      b1r = pick(nok, c1, 16, 32, 1.0, S.ovka + bxo, S.ovk + bxo, S.ovk + byo);
     }
     {
-      const double s = (c2 & 2) ? -1.0 : 1.0;
-      a2r.p1 = S.xv + ((PB * V + PA) * O + HS) * V;  // (hs pA|p7 pB)
-      a2r.p2 = S.xv + ((PA * V + PB) * O + HS) * V;  // (hs pB|p7 pA)
-      a2r.c1 = (mok && (c2 & 4)) ? s : 0.0;
-      a2r.c2 = (mok && (c2 & 8)) ? -s : 0.0;
-      b2r.p1 = S.t2p + ((HA * O + HB) * V + PS) * V;  // t(ps,p7,hA,hB)
-      b2r.p2 = S.t2p + ((HB * O + HA) * V + PS) * V;  // t(p7,ps,hA,hB)
-      b2r.c1 = (nok && (c2 & 16)) ? 1.0 : 0.0;
-      b2r.c2 = (nok && (c2 & 32)) ? -1.0 : 0.0;
+      const double sg = (c2 & 2) ? -1.0 : 1.0;
+      // (hs pA|p7 pB) - (hs pB|p7 pA) = xv[pB,pA,hs,:] - xv[pA,pB,hs,:]
+      const size_t xo = ((PB * V + PA) * O + HS) * Vp, yo = ((PA * V + PB) * O + HS) * Vp;
+      a2r = pick(mok, c2, 4, 8, sg, S.xva + xo, S.xv + xo, S.xv + yo);
+      // t(ps,p7,hA,hB) - t(p7,ps,hA,hB) = t2p[hA,hB,ps,:] - t2p[hB,hA,ps,:]
+      const size_t bxo = ((HA * O + HB) * V + PS) * Vp, byo = ((HB * O + HA) * V + PS) * Vp;
+      b2r = pick(nok, c2, 16, 32, 1.0, S.t2pa + bxo, S.t2p + bxo, S.t2p + byo);
     }
 
     v4d acc[2][2];
@@ -252,33 +266,27 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, const TaskDev& T,
 #pragma unroll
       for(int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    double    ax[4], ay[4], bx[4], by[4];
-    double    ca1, ca2, cb1, cb2;
-    const int nc = n1 + n2;
+    double    ax[8], bx[8];
+    const int nc    = n1 + n2;
     auto      fetch = [&](int c) {
       if(c < n1) {
-        fetch4(a1r, c * KC + 4 * kq, O, ax, ay);
-        fetch4(b1r, c * KC + 4 * kq, O, bx, by);
+        fetch8(a1r, c * KC + 8 * kq, (int) Op, ax);
+        fetch8(b1r, c * KC + 8 * kq, (int) Op, bx);
       }
       else {
-        fetch4(a2r, (c - n1) * KC + 4 * kq, V, ax, ay);
-        fetch4(b2r, (c - n1) * KC + 4 * kq, V, bx, by);
+        fetch8(a2r, (c - n1) * KC + 8 * kq, (int) Vp, ax);
+        fetch8(b2r, (c - n1) * KC + 8 * kq, (int) Vp, bx);
       }
     };
     fetch(0);
 #pragma unroll 1
     for(int c = 0; c < nc; c++) {
-      if(c < n1) {
-        ca1 = a1r.c1; ca2 = a1r.c2; cb1 = b1r.c1; cb2 = b1r.c2;
-      }
-      else {
-        ca1 = a2r.c1; ca2 = a2r.c2; cb1 = b2r.c1; cb2 = b2r.c2;
-      }
+      const double ca = c < n1 ? a1r.c : a2r.c, cb = c < n1 ? b1r.c : b2r.c;
       __syncthreads();
 #pragma unroll
-      for(int j = 0; j < 4; j++) {
-        As[row * LDK + 4 * kq + j] = ca1 * ax[j] + ca2 * ay[j];
-        Bs[row * LDK + 4 * kq + j] = cb1 * bx[j] + cb2 * by[j];
+      for(int j = 0; j < 8; j++) {
+        As[row * LDK + 8 * kq + j] = ca * ax[j];
+        Bs[row * LDK + 8 * kq + j] = cb * bx[j];
       }
       __syncthreads();
       if(c + 1 < nc) fetch(c + 1);
@@ -294,22 +302,22 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, const TaskDev& T,
   unsigned s1mask = 0;
 #pragma unroll
   for(int slot = 0; slot < 9; slot++)
-    if((T.s1code >> (4 * slot)) & 1) s1mask |= 1u << slot;
+    if((s1code >> (4 * slot)) & 1) s1mask |= 1u << slot;
   for(int x = tid; x < 9 * 16; x += NT) {
     const int      slot = x >> 4, pai = (x >> 2) & 3, hii = x & 3;
-    const unsigned c    = (unsigned) (T.s1code >> (4 * slot)) & 15u;
+    const unsigned c    = (unsigned) (s1code >> (4 * slot)) & 15u;
     const int      pa = slot / 3, hi = slot % 3;
     const int      gp = 4 * sub[3 + pa] + pai, gh = 4 * sub[hi] + hii;
     double         v  = 0.0;
-    if((c & 1) && gp < T.dim[3 + pa] && gh < T.dim[hi]) {
-      v = S.t1[(size_t) (T.off[3 + pa] + gp) * O + T.off[hi] + gh];
+    if((c & 1) && gp < sm.dim[3 + pa] && gh < sm.dim[hi]) {
+      v = S.t1[(size_t) (sm.off[3 + pa] + gp) * O + sm.off[hi] + gh];
       if(c & 2) v = -v;
     }
     T1s[x] = v;
   }
   for(int x = tid; x < 9 * 256; x += NT) {
     const int      slot = x >> 8, w = x & 255;
-    const unsigned c    = (unsigned) (T.s1code >> (4 * slot)) & 15u;
+    const unsigned c    = (unsigned) (s1code >> (4 * slot)) & 15u;
     const int      pa = slot / 3, hi = slot % 3;
     int            hj, hk, pb, pc;
     pair_of(hi, hj, hk);
@@ -317,9 +325,9 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, const TaskDev& T,
     const int gk = 4 * sub[hk] + (w & 3), gj = 4 * sub[hj] + ((w >> 2) & 3),
               gc = 4 * sub[3 + pc] + ((w >> 4) & 3), gb = 4 * sub[3 + pb] + (w >> 6);
     double v = 0.0;
-    if((c & 1) && gk < T.dim[hk] && gj < T.dim[hj] && gc < T.dim[3 + pc] && gb < T.dim[3 + pb]) {
-      const size_t K = T.off[hk] + gk, J = T.off[hj] + gj, PC = T.off[3 + pc] + gc,
-                   PBx = T.off[3 + pb] + gb;
+    if((c & 1) && gk < sm.dim[hk] && gj < sm.dim[hj] && gc < sm.dim[3 + pc] && gb < sm.dim[3 + pb]) {
+      const size_t K = sm.off[hk] + gk, J = sm.off[hj] + gj, PC = sm.off[3 + pc] + gc,
+                   PBx = sm.off[3 + pb] + gb;
       const double x1 = (c & 4) ? S.vvoo[((PC * V + PBx) * O + K) * O + J] : 0.0;  // (hk pc|hj pb)
       const double x2 = (c & 8) ? S.vvoo[((PBx * V + PC) * O + K) * O + J] : 0.0;  // (hk pb|hj pc)
       v               = ((c & 4) && (c & 8)) ? x1 - x2 : ((c & 4) ? x1 : -x2);
@@ -327,71 +335,111 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, const TaskDev& T,
     V1s[x] = v;
   }
   __syncthreads();
-  subtile_energy(sm, sub, T.dim, s1mask, e4, e5);
+  subtile_energy(sm, sm.sub, sm.dim, s1mask, e4, e5);
 }
 
-__device__ __forceinline__ void decode_sub(const TaskDev& T, int q, int* sub) {
-#pragma unroll
-  for(int p = 0; p < 6; p++) {
-    sub[p] = q % T.nsub[p];
-    q /= T.nsub[p];
+// Adds a double to a 128-bit two's-complement fixed-point accumulator at
+// scale 2^64 (acc[0] low word, acc[1] high word).  Integer addition is
+// associative, so the final value does not depend on the order of the adds:
+// every task's energy is bit-identical under any schedule.  |p| < 2^53.
+__device__ __forceinline__ void fx_add(unsigned long long* acc, double p) {
+  const bool         neg  = p < 0.0;
+  const double       a    = fabs(p);
+  unsigned long long hi   = (unsigned long long) a;  // floor
+  const double       frac = a - (double) hi;         // exact
+  unsigned long long lo   = (unsigned long long) (frac * 18446744073709551616.0);
+  if(neg) {
+    lo = ~lo + 1ull;
+    hi = ~hi + (lo == 0ull ? 1ull : 0ull);
   }
+  const unsigned long long old   = atomicAdd(&acc[0], lo);
+  const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
+  atomicAdd(&acc[1], hi + carry);
 }
 
+__device__ __forceinline__ double fx_value(const unsigned long long* acc) {
+  return (double) (long long) acc[1] + (double) acc[0] * 5.421010862427522170037264e-20;  // 2^-64
+}
+
+// Persistent kernel: workgroup b serves queue b % kQueues first (dispatch is
+// round-robin over the 8 XCDs, so that is its own XCD's queue), then steals
+// from the others.  One ticket = one sub-tile.
 __global__ void __launch_bounds__(NT) k_spatial_fp64(SpatialDev S, RunDev R) {
-  __shared__ Smem sm;
-  __shared__ int  s_unit;
+  __shared__ Smem      sm;
+  __shared__ int       s_task;
+  __shared__ long long s_local;
+  const int            tid  = threadIdx.x;
+  const int            home = blockIdx.x % kQueues;
 #pragma unroll 1
-  for(;;) {
-    __syncthreads();
-    if(threadIdx.x == 0) {
-      const unsigned t = atomicAdd(R.queue, 1u);
-      s_unit           = (int) t < R.n_units ? (R.order ? R.order[t] : (int) t) : -1;
-    }
-    __syncthreads();
-    const int unit = s_unit;
-    if(unit < 0) break;
-    const int     ti = R.unit_task[unit];
-    const TaskDev T  = R.tasks[ti];
-    if(R.task_t0 && threadIdx.x == 0) {
-      const unsigned long long now = wall_clock64();
-      atomicMin(&R.task_t0[ti], now);
-    }
-    const int q0 = (unit - T.unit_begin) * T.subs_per_unit;
-    const int q1 = min(q0 + T.subs_per_unit, T.n_sub);
-    double    u4 = 0.0, u5 = 0.0;
+  for(int qi = 0; qi < kQueues; qi++) {
+    const int q = (home + qi) % kQueues;
 #pragma unroll 1
-    for(int q = q0; q < q1; q++) {
-      int sub[6];
-      decode_sub(T, q, sub);
-      double e4, e5;
-      spatial_subtile(sm, S, T, sub, e4, e5);
-      if(threadIdx.x == 0) {
-        u4 += e4;
-        u5 += e5;
+    for(;;) {
+      __syncthreads();  // previous sub-tile finished with LDS and s_task
+      if(tid == 0) {
+        const unsigned long long t = atomicAdd(&R.qctr[q], 1ull);
+        if((long long) t >= R.qtotal[q]) s_task = -1;
+        else {
+          int lo = R.qbeg[q], hi = R.qbeg[q + 1] - 1;  // last entry with qtick <= t
+          while(lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if(R.qtick[mid] <= (long long) t) lo = mid;
+            else hi = mid - 1;
+          }
+          s_task  = R.qtask[lo];
+          s_local = (long long) t - R.qtick[lo];
+        }
+      }
+      __syncthreads();
+      const int ti = s_task;
+      if(ti < 0) break;
+      const TaskDev& T = R.tasks[ti];
+      // ticket -> sub-tile: low 6 bits = position inside the 2^6 box
+      long long box = s_local >> 6;
+      const int inb = (int) (s_local & 63);
+      bool      ok  = true;
+      int       sub[6];
+#pragma unroll
+      for(int p = 0; p < 6; p++) {
+        const int nb = T.nbox[p];
+        sub[p]       = 2 * (int) (box % nb) + ((inb >> p) & 1);
+        box /= nb;
+        ok = ok && sub[p] < T.nsub[p];
+      }
+      if(!ok) continue;
+#pragma unroll
+      for(int p = 0; p < 6; p++)
+        if(tid == p) {
+          sm.sub[p] = sub[p];
+          sm.dim[p] = T.dim[p];
+          sm.off[p] = T.off[p];
+        }
+      __syncthreads();
+      const unsigned long long t0 = wall_clock64();
+      double                   e4, e5;
+      spatial_subtile(sm, S, T.d1code, T.d2code, T.s1code, e4, e5);
+      if(tid == 0) {
+        fx_add(&R.acc[4 * (size_t) ti], e4);
+        fx_add(&R.acc[4 * (size_t) ti + 2], e5);
+        if(R.cost) atomicAdd(&R.cost[ti], wall_clock64() - t0);
       }
     }
-    if(threadIdx.x == 0) {
-      R.unit_e[2 * (size_t) unit]     = u4;
-      R.unit_e[2 * (size_t) unit + 1] = u5;
-      if(R.task_t1) atomicMax(&R.task_t1[ti], wall_clock64());
-    }
   }
 }
 
-// Per-task fixed-order sum of unit partials, scaled by the task factor.
-__global__ void k_reduce_tasks(const TaskDev* tasks, int n_tasks, const double* unit_e,
+// Per-task energy from the fixed-point accumulators, scaled by the task factor.
+__global__ void k_reduce_tasks(const TaskDev* tasks, int n_tasks, const unsigned long long* acc,
                                double* task_e) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if(t >= n_tasks) return;
-  const TaskDev& T = tasks[t];
-  double         a = 0.0, b = 0.0;
-  for(int u = T.unit_begin; u < T.unit_begin + T.n_units; u++) {
-    a += unit_e[2 * (size_t) u];
-    b += unit_e[2 * (size_t) u + 1];
-  }
-  task_e[2 * (size_t) t]     = T.factor * a;
-  task_e[2 * (size_t) t + 1] = T.factor * b;
+  const double f             = tasks[t].factor;
+  task_e[2 * (size_t) t]     = f * fx_value(&acc[4 * (size_t) t]);
+  task_e[2 * (size_t) t + 1] = f * fx_value(&acc[4 * (size_t) t + 2]);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, long n) {
+  const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) p[i] = 0ull;
 }
 
 // Fixed-shape sum over tasks in task order (one workgroup of NT threads).
@@ -439,8 +487,8 @@ __global__ void k_transpose(const double* __restrict__ in, double* __restrict__ 
   }
 }
 
-static int transpose(const double* in, double* out, long rows, long cols, long lda, long batch,
-                     long in_bs, long out_bs, hipStream_t st) {
+static int transpose(const double* in, double* out, long rows, long cols, long lda, long ldo,
+                     long batch, long in_bs, long out_bs, hipStream_t st) {
   // grid.y and grid.z are limited to 65535
   const long max_yz = 65535;
   for(long z0 = 0; z0 < batch; z0 += max_yz) {
@@ -449,32 +497,90 @@ static int transpose(const double* in, double* out, long rows, long cols, long l
       const long nr = std::min(32 * max_yz, rows - r0);
       dim3       grid((unsigned) ((cols + 31) / 32), (unsigned) ((nr + 31) / 32), (unsigned) nz);
       hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, st, in + z0 * in_bs + r0 * lda,
-                         out + z0 * out_bs + r0, nr, cols, lda, rows, in_bs, out_bs);
+                         out + z0 * out_bs + r0, nr, cols, lda, ldo, in_bs, out_bs);
       if(hipGetLastError() != hipSuccess) return (int) hipErrorLaunchFailure;
     }
   }
   return 0;
 }
 
-int launch_relayout(const double* t2_in, const double* ovoo_in, const double* vvvo_in, int O,
-                    int V, double* t2h, double* t2p, double* ovk, double* xv, void* stream) {
+// out[r][k] = in[r][k] for k < K, 0 for K <= k < Kp
+__global__ void k_pad_rows(double* __restrict__ out, const double* __restrict__ in, long rows,
+                           long K, long Kp) {
+  const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= rows * Kp) return;
+  const long r = i / Kp, k = i % Kp;
+  out[i]       = k < K ? in[r * K + k] : 0.0;
+}
+
+// out[i0][i1][i2][k] = in[i0][i1][i2][k] - in[j0][j1][j2][k] with (j0,j1,j2) =
+// (i1,i0,i2) (swap02 = 0) or (i2,i1,i0) (swap02 = 1); padding stays zero.
+__global__ void k_antisym(double* __restrict__ out, const double* __restrict__ in, long n0,
+                          long n1, long n2, long Kp, int swap02) {
+  const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n0 * n1 * n2 * Kp) return;
+  long       r  = i / Kp;
+  const long k  = i % Kp;
+  const long i2 = r % n2;
+  r /= n2;
+  const long i1 = r % n1, i0 = r / n1;
+  const long j  = swap02 ? ((i2 * n1 + i1) * n2 + i0) * Kp + k : ((i1 * n1 + i0) * n2 + i2) * Kp + k;
+  out[i]        = in[i] - in[j];
+}
+
+static unsigned nblk(long n) { return (unsigned) ((n + 255) / 256); }
+
+int build_operands(int O, int V, double* t2, double* ovoo, double*& vvvo, SpatialBufs& B,
+                   void* stream) {
   hipStream_t st = (hipStream_t) stream;
-  const long  o = O, v = V;
-  hipError_t  err;
-  // t2h = t2 as given, ovk = ovoo as given
-  if(t2h != t2_in &&
-     (err = hipMemcpyAsync(t2h, t2_in, sizeof(double) * v * v * o * o, hipMemcpyDeviceToDevice, st)))
-    return (int) err;
-  if(ovk != ovoo_in &&
-     (err = hipMemcpyAsync(ovk, ovoo_in, sizeof(double) * o * v * o * o, hipMemcpyDeviceToDevice, st)))
-    return (int) err;
-  // t2p[i][j][a][b] = t2[a][b][i][j]: transpose of a (V*V) x (O*O) matrix
-  int rc = transpose(t2_in, t2p, v * v, o * o, o * o, 1, 0, 0, st);
-  if(rc) return rc;
-  // xv[c][b][i][a] = vvvo[a][c][b][i]: for each c, transpose rows a (stride V*V*O)
-  // x columns (b,i) (V*O contiguous) into out[c][(b,i)][a].
-  rc = transpose(vvvo_in, xv, v, v * o, v * v * o, v, v * o, v * o * v, st);
-  return rc;
+  const long  o = O, v = V, op = (O + 7) / 8 * 8, vp = (V + 7) / 8 * 8;
+  auto        alloc = [](double** p, long n) { return hipMalloc((void**) p, sizeof(double) * n); };
+  hipError_t  e;
+  int         rc;
+  // d1 operands (contraction over h7, padded to op)
+  if((e = alloc(&B.t2h, v * v * o * op))) return (int) e;
+  hipLaunchKernelGGL(k_pad_rows, dim3(nblk(v * v * o * op)), dim3(256), 0, st, B.t2h, t2, v * v * o,
+                     o, op);
+  if((e = alloc(&B.t2ha, v * v * o * op))) return (int) e;
+  hipLaunchKernelGGL(k_antisym, dim3(nblk(v * v * o * op)), dim3(256), 0, st, B.t2ha, B.t2h, v, v,
+                     o, op, 0);
+  if((e = alloc(&B.ovk, o * v * o * op))) return (int) e;
+  hipLaunchKernelGGL(k_pad_rows, dim3(nblk(o * v * o * op)), dim3(256), 0, st, B.ovk, ovoo,
+                     o * v * o, o, op);
+  if((e = alloc(&B.ovka, o * v * o * op))) return (int) e;
+  hipLaunchKernelGGL(k_antisym, dim3(nblk(o * v * o * op)), dim3(256), 0, st, B.ovka, B.ovk, o, v,
+                     o, op, 1);
+  // d2 N side: t2p[i][j][a][b] = t2[a][b][i][j], padded in b
+  double* tmp = nullptr;
+  if((e = alloc(&tmp, o * o * v * v))) return (int) e;
+  if((rc = transpose(t2, tmp, v * v, o * o, o * o, v * v, 1, 0, 0, st))) return rc;
+  if((e = alloc(&B.t2p, o * o * v * vp))) return (int) e;
+  hipLaunchKernelGGL(k_pad_rows, dim3(nblk(o * o * v * vp)), dim3(256), 0, st, B.t2p, tmp, o * o * v,
+                     v, vp);
+  if((e = alloc(&B.t2pa, o * o * v * vp))) return (int) e;
+  hipLaunchKernelGGL(k_antisym, dim3(nblk(o * o * v * vp)), dim3(256), 0, st, B.t2pa, B.t2p, o, o, v,
+                     vp, 0);
+  if((e = hipStreamSynchronize(st))) return (int) e;
+  (void) hipFree(tmp);
+  // d2 M side: xv[c][b][i][a] = vvvo[a][c][b][i], padded in a
+  if((e = alloc(&B.xv, v * v * o * vp))) return (int) e;
+  if((e = hipMemsetAsync(B.xv, 0, sizeof(double) * v * v * o * vp, st))) return (int) e;
+  if((rc = transpose(vvvo, B.xv, v, v * o, v * v * o, vp, v, v * o, v * o * vp, st))) return rc;
+  if((e = hipStreamSynchronize(st))) return (int) e;
+  (void) hipFree(vvvo);
+  vvvo = nullptr;
+  if((e = alloc(&B.xva, v * v * o * vp))) return (int) e;
+  hipLaunchKernelGGL(k_antisym, dim3(nblk(v * v * o * vp)), dim3(256), 0, st, B.xva, B.xv, v, v, o,
+                     vp, 0);
+  if((e = hipGetLastError())) return (int) e;
+  return (int) hipStreamSynchronize(st);
+}
+
+int launch_zero_u64(unsigned long long* p, long n, void* stream) {
+  if(n <= 0) return 0;
+  hipLaunchKernelGGL(k_zero_u64, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t) stream, p, n);
+  return (int) hipGetLastError();
 }
 
 int launch_spatial_fp64(const SpatialDev& S, const RunDev& R, int grid, void* stream) {
@@ -482,11 +588,11 @@ int launch_spatial_fp64(const SpatialDev& S, const RunDev& R, int grid, void* st
   return (int) hipGetLastError();
 }
 
-int launch_reduce(const TaskDev* tasks, int n_tasks, const double* unit_e, double* task_e,
-                  double* e45, void* stream) {
+int launch_reduce(const TaskDev* tasks, int n_tasks, const unsigned long long* acc,
+                  double* task_e, double* e45, void* stream) {
   hipStream_t st = (hipStream_t) stream;
   hipLaunchKernelGGL(k_reduce_tasks, dim3((n_tasks + 255) / 256), dim3(256), 0, st, tasks,
-                     n_tasks, unit_e, task_e);
+                     n_tasks, acc, task_e);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(NT), 0, st, task_e, n_tasks, e45);
   return (int) hipGetLastError();
 }
@@ -574,18 +680,18 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
         const int nc = cdiv(K, KC);
 #pragma unroll 1
         for(int c = 0; c < nc; c++) {
-          double av[4], bv[4];
+          double av[8], bv[8];
 #pragma unroll
-          for(int j = 0; j < 4; j++) {
-            const int k = c * KC + 4 * kq + j;
+          for(int j = 0; j < 8; j++) {
+            const int k = c * KC + 8 * kq + j;
             av[j]       = (mok && k < K) ? sg * abase[k * astr] : 0.0;
             bv[j]       = (nok && k < K) ? bbase[k * bstr] : 0.0;
           }
           __syncthreads();
 #pragma unroll
-          for(int j = 0; j < 4; j++) {
-            As[row * LDK + 4 * kq + j] = av[j];
-            Bs[row * LDK + 4 * kq + j] = bv[j];
+          for(int j = 0; j < 8; j++) {
+            As[row * LDK + 8 * kq + j] = av[j];
+            Bs[row * LDK + 8 * kq + j] = bv[j];
           }
           __syncthreads();
           mfma_chunk(As, Bs, acc);
