@@ -1,15 +1,25 @@
 """Benchmark of the closed-shell CCSD(T) triples energy on one MI355X.
 
-One step = one full (T) energy (every task of the workload, one persistent
-launch + the fixed-order reductions) with inputs already resident in HBM.
-Default workload: O=48, V=384 with seeded uneven tiles (occupied U[6,20],
-virtual U[16,48]), synthetic inputs of the stated distributions.
+One step = one full (T) energy over every task of the workload: host queue
+build, one graph launch (fill kernels, persistent kernel, reductions) and the
+copy of E[4]/E[5] back; inputs are resident in HBM before timing starts and
+each step returns only after the stream has drained, so the wall clock around
+it brackets the device work.  Default workload: O=48, V=384, seeded uneven
+tiles (occupied U[6,20], virtual U[16,48]), synthetic inputs of the stated
+distributions (integrals N(0, 0.05^2), T1 N(0, 1e-6), MP2-like T2), mixed
+precision.
 
-Prints ONE JSON line (rank 0).  value = time-to-E(T) in seconds (lower is
-better); roofline uses the analytic (reference) flop count per launch over
-the kernel's HIP-event time; cpu_baseline times the CPU oracle (a C++
-restatement of the reference CPU path) on a bounded sample of the same tasks
-and extrapolates by the flop count.
+Prints ONE JSON line.  value = time-to-E(T) in seconds (lower is better).
+roofline: the reference's analytic flop count per launch over the persistent
+kernel's HIP-event time.  Beside it:
+  * fp64_compare: the plain FP64 kernel and the measured precision on the same
+    seeded task sample (or on the whole workload with --fp64-full): time ratio
+    and |dE4|, |dE5| of the measured precision against FP64;
+  * cpu_baseline: the CPU oracle (a C++ restatement of the reference's CPU
+    fused (T) path) timed on a bounded sample of the workload's tasks,
+    extrapolated by the flop count; its energies on those tasks are the
+    reference values for oracle_check (GPU FP64 and measured precision vs the
+    oracle, task by task).
 """
 import argparse
 import json
@@ -22,7 +32,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"fp64": 78.6, "mixed": 2500.0}   # MI355X dense matrix peaks (FP64, BF16/FP16)
+PEAK_TFLOPS = {"fp64": 78.6, "mixed": 2500.0}   # MI355X dense matrix peaks (FP64, FP16)
 SEED = 20241202
 
 CONFIGS = {
@@ -41,26 +51,31 @@ def make_tiling(xt, name):
     return xt.Tiling.uneven(O_, V_, spec[1], spec[2], seed=SEED)
 
 
-def cpu_baseline(xt, tiling, s, per_task, budget_s):
-    """Times the oracle on tasks of the workload, smallest first, until the
-    budget is spent; returns the flop rate and the sample description."""
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(tiling, s, per_task, budget_s):
+    """Times the oracle on tasks of the workload, from the 25th flop
+    percentile upwards, until the budget is spent."""
     from oracle import oracle as O
     O.build()
     occ, vir = tiling.occ, tiling.vir
     t = O.Tiling(occ, occ, vir, vir)
     evl = np.concatenate([s["eo"], s["eo"], s["ev"], s["ev"]])
-    # tasks in increasing size from the 25th percentile on, until the budget is spent
     order = np.argsort(per_task, kind="stable")
-    pick, spent, flops = [], 0.0, 0.0
+    pick, parts, spent, flops = [], [], 0.0, 0.0
     for k in order[len(order) // 4:]:
         t0 = time.time()
-        O.run_spatial(t, evl, s, True, [int(k)])
+        _, _, pt = O.run_spatial(t, evl, s, True, [int(k)])
         spent += time.time() - t0
         flops += per_task[k]
         pick.append(int(k))
+        parts.append(pt[0])
         if spent > budget_s:
             break
-    return flops / spent, spent, pick, flops, O.num_threads()
+    return dict(rate=flops / spent, spent=spent, pick=pick, flops=flops,
+                threads=O.num_threads(), partials=np.asarray(parts))
 
 
 def main():
@@ -69,23 +84,30 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--schedule", default="analytic", choices=["driver", "analytic", "learned"])
+    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of the graph")
     ap.add_argument("--task-sample", type=int, default=0,
-                    help="profiling aid: run only this many tasks (not the judged workload)")
+                    help="profiling aid: time only this many tasks (not the benchmark workload)")
+    ap.add_argument("--compare-frac", type=float, default=0.04,
+                    help="fraction of tasks in the seeded FP64 comparison sample")
+    ap.add_argument("--fp64-full", action="store_true",
+                    help="compare against a full FP64 run instead of a task sample")
+    ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding of the task list is a design note, not built)",
-              file=sys.stderr)
+              "(sharding the task list over GPUs is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
     import exachem_amd as xt
-    prec = {"fp64": xt.PREC_FP64, "mixed": xt.PREC_MIXED}[args.precision]
+    PREC = {"fp64": xt.PREC_FP64, "mixed": xt.PREC_MIXED}
+    prec = PREC[args.precision]
     sched = {"driver": xt.SCHED_DRIVER, "analytic": xt.SCHED_ANALYTIC,
              "learned": xt.SCHED_LEARNED}[args.schedule]
+    graph = 0 if args.no_graph else 1
     tiling = make_tiling(xt, args.config)
     O_, V_ = tiling.O, tiling.V
     t0 = time.time()
@@ -93,33 +115,37 @@ def main():
     ctx = xt.CCSDTriples(0)
     ctx.set_tiling(tiling)
     ctx.upload(s)
-    prep_s = time.time() - t0
+    upload_s = time.time() - t0
     total_flops, per_task, parts = ctx.flops()
     n_tasks = ctx.num_tasks()
+    # operand conversion of the chosen precision (one-time, like the upload)
+    t0 = time.time()
+    ctx.prepare(prec)
+    operand_s = time.time() - t0
     which = None
     if args.task_sample:
         rng = np.random.default_rng(1)
         which = np.sort(rng.choice(n_tasks, min(args.task_sample, n_tasks), replace=False))
         total_flops = float(per_task[which].sum())
-    print(f"[bench] {args.config} O={O_} V={V_} tasks={n_tasks} flops={total_flops:.4e} "
-          f"prep={prep_s:.1f}s occ={tiling.occ} vir={tiling.vir}", file=sys.stderr, flush=True)
+    log(f"{args.config} O={O_} V={V_} tasks={n_tasks} flops={total_flops:.4e} "
+        f"upload={upload_s:.1f}s operands={operand_s:.1f}s occ={tiling.occ} vir={tiling.vir}")
 
     def step():
         if which is None:
-            return ctx.run(precision=prec, schedule=sched, partials=False)
-        return ctx.run_tasks(which, precision=prec, schedule=sched)
+            return ctx.run(precision=prec, schedule=sched, use_graph=graph, partials=False)
+        return ctx.run_tasks(which, precision=prec, schedule=sched, use_graph=graph)
 
     for w in range(args.warmup):
         step()
-        print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
+        log(f"warmup {w} done")
     times, kms, res = [], [], None
     for k in range(args.steps):
         t0 = time.perf_counter()
         res = step()
         times.append(time.perf_counter() - t0)
         kms.append(res["kernel_ms"])
-        print(f"[bench] step {k}: {times[-1]:.3f}s kernel {res['kernel_ms']:.1f} ms "
-              f"E4={res['e4']!r} E5={res['e5']!r}", file=sys.stderr, flush=True)
+        log(f"step {k}: {times[-1]:.3f}s kernel {res['kernel_ms']:.1f} ms "
+            f"E4={res['e4']!r} E5={res['e5']!r}")
     sec = float(np.mean(times))
     kernel_s = float(np.mean(kms)) / 1e3
     achieved = total_flops / kernel_s / 1e12
@@ -128,28 +154,71 @@ def main():
         "metric": "time-to-E(T) + MFMA TFLOP/s at O=48,V=384; |dE(T)| vs FP64 oracle",
         "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "strong",
-        "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "bf16x3/f32/f64",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f16x3 products / f32 accum / f64 energy",
         "data": "synthetic (seeded generator, stated distributions)",
-        "config": {"workload": f"closed-shell (T) O={O_} V={V_} "
-                               f"{'uneven' if 'uneven' in str(CONFIGS[args.config][2]) else 'uniform'} tiles",
+        "config": {"workload": f"closed-shell (T) energy, O={O_} V={V_}, "
+                               f"{CONFIGS[args.config][2][0]} tiles, all tasks",
                    "O": O_, "V": V_, "occ_tiles": tiling.occ, "vir_tiles": tiling.vir,
                    "tasks": n_tasks if which is None else int(len(which)),
                    "task_sample": bool(which is not None), "precision": args.precision,
-                   "schedule": args.schedule, "parallelism": "single GPU, persistent work queue"},
+                   "schedule": args.schedule, "graph": bool(graph),
+                   "parallelism": "single GPU, persistent kernel over per-XCD work queues"},
         "E4": res["e4"], "E5": res["e5"], "tflops": achieved,
+        "setup_s": {"upload_and_layout": upload_s, "operand_split": operand_s},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
                      "flops_per_launch": total_flops, "flop_count": "reference analytic count",
                      "kernel_ms": kernel_s * 1e3},
     }
+    if args.precision == "mixed":
+        # three FP16 MFMA products per d1/d2 multiply-add
+        issued = (3 * (parts[1] + parts[2]) + parts[0]) if which is None else 3 * total_flops
+        line["roofline"]["mfma_issued_tflops"] = issued / kernel_s / 1e12
+
+    if not args.no_compare and args.precision != "fp64":
+        if args.fp64_full:
+            cw = np.arange(n_tasks)
+        else:
+            rng = np.random.default_rng(SEED)
+            cw = np.sort(rng.choice(n_tasks, max(1, int(args.compare_frac * n_tasks)),
+                                    replace=False))
+        log(f"comparison on {len(cw)} tasks")
+        f = ctx.run_tasks(cw, precision=xt.PREC_FP64, schedule=sched, use_graph=graph)
+        m = ctx.run_tasks(cw, precision=prec, schedule=sched, use_graph=graph)
+        d4, d5 = abs(m["e4"] - f["e4"]), abs(m["e5"] - f["e5"])
+        line["fp64_compare"] = {
+            "tasks": int(len(cw)), "full": bool(args.fp64_full),
+            "flops": float(per_task[cw].sum()),
+            "fp64_kernel_ms": f["kernel_ms"], "kernel_ms": m["kernel_ms"],
+            "speedup_vs_fp64": f["kernel_ms"] / m["kernel_ms"],
+            "E4_fp64": f["e4"], "E5_fp64": f["e5"],
+            "abs_dE4": d4, "abs_dE5": d5, "abs_dET": d5,
+            "rel_dE4": d4 / abs(f["e4"]), "rel_dE5": d5 / abs(f["e5"]),
+            "max_task_rel_dE4": float(np.max(np.abs(m["task_partials"][:, 0] -
+                                                    f["task_partials"][:, 0]) /
+                                             np.abs(f["task_partials"][:, 0]))),
+        }
+        log(f"fp64 {f['kernel_ms']:.1f} ms, {args.precision} {m['kernel_ms']:.1f} ms, "
+            f"rel dE4 {d4 / abs(f['e4']):.2e}")
+
     if not args.no_cpu_baseline:
-        rate, spent, pick, fl, cores = cpu_baseline(xt, tiling, s, per_task, args.cpu_budget)
+        cb = cpu_baseline(tiling, s, per_task, args.cpu_budget)
         line["cpu_baseline"] = {
-            "value": total_flops / rate, "unit": "s (extrapolated time-to-E(T))",
-            "cores": cores, "kind": "port",
-            "sample": f"{len(pick)} tasks of this workload ({fl:.3e} flops) timed in "
-                      f"{spent:.1f}s = {rate / 1e9:.2f} GFLOP/s, extrapolated by the analytic "
-                      f"flop count"}
+            "value": total_flops / cb["rate"], "unit": "s (extrapolated time-to-E(T))",
+            "cores": cb["threads"], "kind": "port",
+            "sample": f"{len(cb['pick'])} tasks of this workload ({cb['flops']:.3e} flops) "
+                      f"timed in {cb['spent']:.1f}s = {cb['rate'] / 1e9:.2f} GFLOP/s, "
+                      f"extrapolated by the analytic flop count"}
+        chk = {}
+        for name, p in (("fp64", xt.PREC_FP64), (args.precision, prec)):
+            g = ctx.run_tasks(cb["pick"], precision=p)["task_partials"]
+            ref = cb["partials"]
+            chk[name] = {"max_abs_dE4": float(np.max(np.abs(g[:, 0] - ref[:, 0]))),
+                         "max_abs_dE5": float(np.max(np.abs(g[:, 1] - ref[:, 1]))),
+                         "max_rel_dE4": float(np.max(np.abs(g[:, 0] - ref[:, 0]) /
+                                                     np.abs(ref[:, 0])))}
+        line["oracle_check"] = dict(tasks=cb["pick"], **chk)
     print(json.dumps(line), flush=True)
     ctx.close()
 

@@ -72,12 +72,13 @@ def lib():
     L.xt_last_kernel_ms.argtypes = [vp, dp]
     L.xt_run_tasks.argtypes = [vp, C.POINTER(XtOpts), C.c_long, lp, dp, dp, C.POINTER(C.c_float)]
     L.xt_task_codes.argtypes = [vp, C.c_long, C.POINTER(C.c_uint64)]
+    L.xt_prepare.argtypes = [vp, C.c_int]
     L.xt_fully_fused_ccsd_t_gpu.argtypes = (
         [vp, sz] + [sz] * 6 + [vp] * 6 + [ip] * 6 + [sz] * 8 + [vp] * 6 + [vp, vp])
     for name in ("xt_create", "xt_destroy", "xt_set_tiling", "xt_num_tasks", "xt_task_list",
                  "xt_flops", "xt_upload_closed_shell", "xt_synth_upload", "xt_synth_closed_shell",
                  "xt_run", "xt_last_kernel_ms", "xt_fully_fused_ccsd_t_gpu", "xt_run_tasks",
-                 "xt_task_codes"):
+                 "xt_task_codes", "xt_prepare"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -17,6 +17,10 @@ struct TaskDev {
   int       n_sub;       // product of nsub
   int       pad_;
   long long n_tickets;   // 64 * prod(nbox): queue tickets of this task (some empty)
+  // mixed-precision kernel: (4,4,4 | 8,8,8) boxes
+  int       nsubm[6];    // ceil(dim/4) for holes, ceil(dim/8) for particles
+  int       nboxm[6];    // ceil(nsubm/2)
+  long long n_ticketsm;  // 64 * prod(nboxm)
   // Per output bipartition g = 3*hs + ps (hs: single hole position 0..2,
   // ps: single particle position 0..2), 6 bits each:
   //   bit0 enabled, bit1 negative sign, bit2 a1, bit3 a2, bit4 b1, bit5 b2
@@ -55,6 +59,28 @@ struct SpatialBufs {
   double *eo, *ev, *t1, *t2h, *t2ha, *ovk, *ovka, *xv, *xva, *t2p, *t2pa, *vvoo;
 };
 
+// Split FP16 operands of the mixed-precision kernel.  Array j (same order as
+// the FP64 operand arrays: 0 t2h, 1 t2ha, 2 ovk, 3 ovka, 4 xv, 5 xva, 6 t2p,
+// 7 t2pa) holds, for each contraction run (row) r, a power-of-two exponent
+// e[r] and the scaled values y = x * 2^e[r] split as y ~ hi + lo (FP16 each),
+// stored per 8 consecutive k as [hi x8][lo x8]; rows are padded to Kp16 =
+// roundup(K, 16) with zeros.  max |y| of a row lies in [2^13, 2^14).
+constexpr int kOperands = 8;
+struct SplitBufs {
+  unsigned char* v[kOperands];   // [rows][Kp16/8][2][8] halves
+  short*         e[kOperands];   // [rows]
+  long           rows[kOperands];
+};
+struct MixedDev {
+  int                  O, V, Ko, Kv;  // Ko = roundup(O,16), Kv = roundup(V,16)
+  const double*        eo;
+  const double*        ev;
+  const double*        t1;
+  const double*        vvoo;
+  const unsigned char* v[kOperands];
+  const short*         e[kOperands];
+};
+
 // Work queues: one per XCD.  Queue q holds the tasks qtask[qbeg[q]..qbeg[q+1])
 // in processing order; entry e owns tickets [qtick[e], qtick[e] + n_tickets)
 // of its queue.  A ticket is one 4^6 sub-tile; 64 consecutive tickets form a
@@ -86,6 +112,8 @@ int launch_spatial_fp64(const SpatialDev& S, const RunDev& R, int grid, void* st
 int launch_reduce(const TaskDev* tasks, int n_tasks, const unsigned long long* acc,
                   double* task_e, double* e45, void* stream);
 int launch_zero_u64(unsigned long long* p, long n, void* stream);
+int build_split(const SpatialBufs& B, int O, int V, SplitBufs& S, void* stream);
+int launch_spatial_mixed(const MixedDev& M, const RunDev& R, int grid, void* stream);
 
 struct PackedArgs {
   int           dim[6];

@@ -265,11 +265,15 @@ int build_plan(Plan& P, int O, int V, const std::vector<int>& occ, const std::ve
     }
     int nd1, nd2, ns1;
     slot_codes(sp, T.d1code, T.d2code, T.s1code, nd1, nd2, ns1);
-    T.factor    = P.factors[k];
-    T.n_tickets = 64;
+    T.factor     = P.factors[k];
+    T.n_tickets  = 64;
+    T.n_ticketsm = 64;
     for(int p = 0; p < 6; p++) {
-      T.nbox[p] = (T.nsub[p] + 1) / 2;
+      T.nbox[p]  = (T.nsub[p] + 1) / 2;
+      T.nsubm[p] = (T.dim[p] + (p < 3 ? 3 : 7)) / (p < 3 ? 4 : 8);
+      T.nboxm[p] = (T.nsubm[p] + 1) / 2;
       T.n_tickets *= T.nbox[p];
+      T.n_ticketsm *= T.nboxm[p];
     }
     double prod = 1;
     for(int p = 0; p < 6; p++) prod *= T.dim[p];
@@ -297,7 +301,9 @@ struct xt_ctx {
   Plan        plan;
   bool        have_plan = false, have_inputs = false;
   // device inputs (kernel layouts)
-  xt::SpatialBufs in = {};
+  xt::SpatialBufs in    = {};
+  xt::SplitBufs   split = {};  // FP16 split operands (built on the first mixed run)
+  bool            have_split = false;
   // device plan and run state
   xt::TaskDev*        d_tasks  = nullptr;
   int*                d_qtask  = nullptr;
@@ -312,8 +318,29 @@ struct xt_ctx {
   hipEvent_t          ev0 = nullptr, ev1 = nullptr;
   double              last_ms = 0.0;
   std::vector<double> learned_cost;  // per-task predicted cost (learned schedule)
+  // Captured launch sequence (zero fills, persistent kernel, reductions).  It
+  // reads the queues from the device buffers above, so one instantiation
+  // serves every schedule and task subset of the same precision / grid /
+  // timing mode; it is dropped whenever a buffer it names is reallocated.
+  hipGraphExec_t graph = nullptr;
+  struct GraphKey {
+    int         precision = -1, grid = 0, timed = 0;
+    long        n_tasks   = -1;
+    const void* inputs    = nullptr;
+    bool        operator==(const GraphKey& k) const {
+      return precision == k.precision && grid == k.grid && timed == k.timed &&
+             n_tasks == k.n_tasks && inputs == k.inputs;
+    }
+  } graph_key;
+
+  void drop_graph() {
+    if(graph) (void) hipGraphExecDestroy(graph);
+    graph     = nullptr;
+    graph_key = GraphKey();
+  }
 
   void free_inputs() {
+    drop_graph();
     double** ps[] = {&in.eo,  &in.ev,  &in.t1,  &in.t2h, &in.t2ha, &in.ovk,
                      &in.ovka, &in.xv, &in.xva, &in.t2p, &in.t2pa, &in.vvoo};
     for(double** p: ps)
@@ -322,8 +349,20 @@ struct xt_ctx {
         *p = nullptr;
       }
     have_inputs = false;
+    free_split();
+  }
+  void free_split() {
+    drop_graph();
+    for(int j = 0; j < xt::kOperands; j++) {
+      if(split.v[j]) (void) hipFree(split.v[j]);
+      if(split.e[j]) (void) hipFree(split.e[j]);
+      split.v[j] = nullptr;
+      split.e[j] = nullptr;
+    }
+    have_split = false;
   }
   void free_plan_dev() {
+    drop_graph();
     void* ps[] = {d_tasks, d_qtask, d_qbeg, d_qtick, d_qtotal, d_qctr, d_acc, d_cost, d_task_e, d_e45};
     for(void* p: ps)
       if(p) (void) hipFree(p);
@@ -541,7 +580,7 @@ void xt_default_opts(xt_opts* o) {
   std::memset(o, 0, sizeof(*o));
   o->precision        = XT_PREC_FP64;
   o->schedule         = XT_SCHED_DRIVER;
-  o->use_graph        = 0;
+  o->use_graph        = 1;
   o->record_task_time = 0;
   o->workgroups       = 0;
 }
@@ -578,7 +617,7 @@ struct Queues {
 };
 
 static Queues build_queues(const Plan& P, const std::vector<int>& sel, int schedule,
-                           const std::vector<double>* cost) {
+                           const std::vector<double>* cost, bool mixed) {
   const int                     nq = xt::kQueues;
   std::vector<std::vector<int>> per(nq);
   if(schedule == XT_SCHED_DRIVER) {
@@ -602,7 +641,7 @@ static Queues build_queues(const Plan& P, const std::vector<int>& sel, int sched
     for(int t: per[q]) {
       Q.task.push_back(t);
       Q.tick.push_back(tk);
-      tk += P.dev[t].n_tickets;
+      tk += mixed ? P.dev[t].n_ticketsm : P.dev[t].n_tickets;
     }
     Q.beg.push_back((int) Q.task.size());
     Q.total.push_back(tk);
@@ -624,7 +663,8 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
   const std::vector<double>* learned =
     (o.schedule == XT_SCHED_LEARNED && (long) c->learned_cost.size() == nt) ? &c->learned_cost
                                                                             : nullptr;
-  Queues      Q  = build_queues(P, sel, o.schedule, learned);
+  const bool  mixed = o.precision == XT_PREC_MIXED;
+  Queues      Q     = build_queues(P, sel, o.schedule, learned, mixed);
   hipStream_t st = c->stream;
   if(!Q.task.empty()) {
     XT_HIP(hipMemcpyAsync(c->d_qtask, Q.task.data(), sizeof(int) * Q.task.size(),
@@ -636,10 +676,6 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
                         st));
   XT_HIP(hipMemcpyAsync(c->d_qtotal, Q.total.data(), sizeof(long long) * Q.total.size(),
                         hipMemcpyHostToDevice, st));
-  int err = xt::launch_zero_u64(c->d_qctr, xt::kQueues, st);
-  if(!err) err = xt::launch_zero_u64(c->d_acc, 4 * std::max<long>(nt, 1), st);
-  if(!err) err = xt::launch_zero_u64(c->d_cost, std::max<long>(nt, 1), st);
-  if(err) return fail(XT_EHIP, std::string("zero: ") + hipGetErrorString((hipError_t) err));
   xt::SpatialDev S;
   S.O    = P.O;
   S.V    = P.V;
@@ -671,12 +707,68 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
   for(long long t: Q.total) tickets += t;
   int grid = o.workgroups > 0 ? o.workgroups : c->n_cu * 2;
   grid     = (int) std::max<long long>(1, std::min<long long>(grid, tickets));
+  xt::MixedDev M = {};
+  if(mixed) {
+    M.O    = P.O;
+    M.V    = P.V;
+    M.Ko   = (P.O + 15) / 16 * 16;
+    M.Kv   = (P.V + 15) / 16 * 16;
+    M.eo   = c->in.eo;
+    M.ev   = c->in.ev;
+    M.t1   = c->in.t1;
+    M.vvoo = c->in.vvoo;
+    for(int j = 0; j < xt::kOperands; j++) {
+      M.v[j] = c->split.v[j];
+      M.e[j] = c->split.e[j];
+    }
+    grid = o.workgroups > 0 ? o.workgroups : c->n_cu;
+    grid = (int) std::max<long long>(1, std::min<long long>(grid, tickets));
+  }
+  // the launch sequence; counters and accumulators are cleared by fill
+  // kernels (no memset nodes when captured)
+  auto body = [&](bool events) -> int {
+    int err = xt::launch_zero_u64(c->d_qctr, xt::kQueues, st);
+    if(!err) err = xt::launch_zero_u64(c->d_acc, 4 * std::max<long>(nt, 1), st);
+    if(!err) err = xt::launch_zero_u64(c->d_cost, std::max<long>(nt, 1), st);
+    if(err) return fail(XT_EHIP, std::string("zero: ") + hipGetErrorString((hipError_t) err));
+    if(events) XT_HIP(hipEventRecord(c->ev0, st));
+    err = mixed ? xt::launch_spatial_mixed(M, R, grid, st) : xt::launch_spatial_fp64(S, R, grid, st);
+    if(err) return fail(XT_EHIP, std::string("launch: ") + hipGetErrorString((hipError_t) err));
+    if(events) XT_HIP(hipEventRecord(c->ev1, st));
+    err = xt::launch_reduce(c->d_tasks, (int) nt, c->d_acc, c->d_task_e, c->d_e45, st);
+    if(err) return fail(XT_EHIP, std::string("reduce: ") + hipGetErrorString((hipError_t) err));
+    return XT_OK;
+  };
+  if(!o.use_graph) return body(true);
+  xt_ctx::GraphKey key;
+  key.precision = o.precision;
+  key.grid      = grid;
+  key.timed     = o.record_task_time ? 1 : 0;
+  key.n_tasks   = nt;
+  key.inputs    = mixed ? (const void*) c->split.v[0] : (const void*) c->in.xv;
+  if(!c->graph || !(c->graph_key == key)) {
+    c->drop_graph();
+    XT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int        rc = body(false);
+    hipGraph_t g  = nullptr;
+    hipError_t e  = hipStreamEndCapture(st, &g);
+    if(rc) {
+      if(g) (void) hipGraphDestroy(g);
+      return rc;
+    }
+    if(e != hipSuccess) return fail(XT_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    (void) hipGraphDestroy(g);
+    if(e != hipSuccess) {
+      c->graph = nullptr;
+      return fail(XT_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    c->graph_key = key;
+  }
+  // with a graph the timed span also holds the two small fill and reduce kernels
   XT_HIP(hipEventRecord(c->ev0, st));
-  err = xt::launch_spatial_fp64(S, R, grid, st);
-  if(err) return fail(XT_EHIP, std::string("launch: ") + hipGetErrorString((hipError_t) err));
+  XT_HIP(hipGraphLaunch(c->graph, st));
   XT_HIP(hipEventRecord(c->ev1, st));
-  err = xt::launch_reduce(c->d_tasks, (int) nt, c->d_acc, c->d_task_e, c->d_e45, st);
-  if(err) return fail(XT_EHIP, std::string("reduce: ") + hipGetErrorString((hipError_t) err));
   return XT_OK;
 }
 
@@ -693,12 +785,30 @@ static int task_times(xt_ctx* c, long nt, float* task_ms) {
 static int check_run_state(xt_ctx* c, const xt_opts& o) {
   if(c->device < 0) return fail(XT_ENODEV, "xt_run: host-only context");
   if(!c->have_plan || !c->have_inputs) return fail(XT_ESTATE, "xt_run: tiling or inputs missing");
-  if(o.precision != XT_PREC_FP64)
-    return fail(XT_EINVAL, "xt_run: precision not supported by this build");
+  if(o.precision != XT_PREC_FP64 && o.precision != XT_PREC_MIXED)
+    return fail(XT_EINVAL, "xt_run: unknown precision");
+  if(o.precision == XT_PREC_MIXED && !c->have_split) {
+    XT_HIP(hipSetDevice(c->device));
+    int err = xt::build_split(c->in, c->plan.O, c->plan.V, c->split, c->stream);
+    if(err) {
+      c->free_split();
+      return fail(err == (int) hipErrorOutOfMemory ? XT_ENOMEM : XT_EHIP,
+                  std::string("split operands: ") + hipGetErrorString((hipError_t) err));
+    }
+    c->have_split = true;
+  }
   if(o.schedule < XT_SCHED_DRIVER || o.schedule > XT_SCHED_LEARNED)
     return fail(XT_EINVAL, "xt_run: bad schedule");
   XT_HIP(hipSetDevice(c->device));
   return ensure_plan_dev(c);
+}
+
+int xt_prepare(xt_ctx* c, int precision) {
+  if(!c) return fail(XT_EINVAL, "xt_prepare: null context");
+  xt_opts o;
+  xt_default_opts(&o);
+  o.precision = precision;
+  return check_run_state(c, o);
 }
 
 int xt_run(xt_ctx* c, const xt_opts* opts, double* e45, double* task_partials, float* task_ms) {

@@ -591,7 +591,7 @@ int launch_spatial_fp64(const SpatialDev& S, const RunDev& R, int grid, void* st
 int launch_reduce(const TaskDev* tasks, int n_tasks, const unsigned long long* acc,
                   double* task_e, double* e45, void* stream) {
   hipStream_t st = (hipStream_t) stream;
-  hipLaunchKernelGGL(k_reduce_tasks, dim3((n_tasks + 255) / 256), dim3(256), 0, st, tasks,
+  hipLaunchKernelGGL(k_reduce_tasks, dim3(n_tasks > 0 ? (n_tasks + 255) / 256 : 1), dim3(256), 0, st, tasks,
                      n_tasks, acc, task_e);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(NT), 0, st, task_e, n_tasks, e45);
   return (int) hipGetLastError();

@@ -149,6 +149,10 @@ class CCSDTriples:
     def synth_upload(self, seed, sigma=0.05, t1_sigma=1e-3):
         check(lib().xt_synth_upload(self._ctx, seed, sigma, t1_sigma))
 
+    def prepare(self, precision):
+        """Builds the operand form of `precision` ahead of the first run."""
+        check(lib().xt_prepare(self._ctx, precision))
+
     def run(self, precision=PREC_FP64, schedule=SCHED_DRIVER, use_graph=0, record_task_time=0,
             workgroups=0, partials=True):
         n = self.num_tasks()
@@ -165,32 +169,30 @@ class CCSDTriples:
                     task_partials=tp[:2 * n].reshape(n, 2) if tp is not None else None,
                     task_ms=tm[:n] if tm is not None else None, kernel_ms=ms.value)
 
+    def run_tasks(self, which, precision=PREC_FP64, schedule=SCHED_DRIVER, use_graph=0,
+                  record_task_time=0, workgroups=0):
+        """Runs the listed tasks only; partials come back in listed order."""
+        which = np.ascontiguousarray(which, dtype=np.int64)
+        n = len(which)
+        o = default_opts(precision=precision, schedule=schedule, use_graph=use_graph,
+                         record_task_time=record_task_time, workgroups=workgroups)
+        e = np.zeros(2)
+        tp = np.zeros(2 * n)
+        tm = np.zeros(n, dtype=np.float32)
+        check(lib().xt_run_tasks(self._ctx, C.byref(o), n,
+                                 which.ctypes.data_as(C.POINTER(C.c_long)), _dp(e), _dp(tp),
+                                 tm.ctypes.data_as(C.POINTER(C.c_float))))
+        ms = C.c_double()
+        check(lib().xt_last_kernel_ms(self._ctx, C.byref(ms)))
+        return dict(e4=float(e[0]), e5=float(e[1]), task_partials=tp.reshape(n, 2),
+                    task_ms=tm if record_task_time else None, kernel_ms=ms.value)
 
-def _run_tasks(self, which, precision=PREC_FP64, schedule=SCHED_DRIVER, record_task_time=0,
-               workgroups=0):
-    which = np.ascontiguousarray(which, dtype=np.int64)
-    n = len(which)
-    o = default_opts(precision=precision, schedule=schedule, record_task_time=record_task_time,
-                     workgroups=workgroups)
-    e = np.zeros(2)
-    tp = np.zeros(2 * n)
-    tm = np.zeros(n, dtype=np.float32)
-    check(lib().xt_run_tasks(self._ctx, C.byref(o), n, which.ctypes.data_as(C.POINTER(C.c_long)),
-                             _dp(e), _dp(tp), tm.ctypes.data_as(C.POINTER(C.c_float))))
-    ms = C.c_double()
-    check(lib().xt_last_kernel_ms(self._ctx, C.byref(ms)))
-    return dict(e4=float(e[0]), e5=float(e[1]), task_partials=tp.reshape(n, 2),
-                task_ms=tm if record_task_time else None, kernel_ms=ms.value)
+    def task_codes(self, task):
+        """(d1code, d2code, s1code) bit fields of one task."""
+        codes = (C.c_uint64 * 3)()
+        check(lib().xt_task_codes(self._ctx, task, codes))
+        return tuple(int(x) for x in codes)
 
-
-def _task_codes(self, task):
-    codes = (C.c_uint64 * 3)()
-    check(lib().xt_task_codes(self._ctx, task, codes))
-    return tuple(int(x) for x in codes)
-
-
-CCSDTriples.run_tasks = _run_tasks
-CCSDTriples.task_codes = _task_codes
 
 
 def fully_fused_ccsd_t_gpu(stream, num_blocks, dims, dev_bufs, meta, noab, nvab, max_dims, evl,

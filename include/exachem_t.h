@@ -155,6 +155,12 @@ typedef struct xt_opts {
 /* Default options (FP64, driver order, graph on, timing off). */
 void xt_default_opts(xt_opts* o);
 
+/* Builds the device operand form a precision needs (for XT_PREC_MIXED the
+ * row-scaled FP16 hi/lo split of every contraction operand), which xt_run
+ * otherwise builds on its first call at that precision.  No kernel of the
+ * energy itself runs. */
+int xt_prepare(xt_ctx* ctx, int precision);
+
 /*
  * Runs every task.  e45[0] = E[4] ([T] correction), e45[1] = E[5] ((T)
  * correction, which includes E[4]), summed in task order in FP64 with the

@@ -747,8 +747,12 @@ void task_kernel_cpu(const SOProblem& P, const Sizes& S, const size_t* t, double
               }
   }
 
-  // energies, ccsd_t_all_fused_cpu.hpp:586-623 (serial)
-  double final_energy_1 = 0.0, final_energy_2 = 0.0;
+  // energies, ccsd_t_all_fused_cpu.hpp:586-623 (serial, same order).  The
+  // reference carries these sums in double; here they are carried in 80-bit
+  // long double so that the oracle's value is the exactly rounded sum of the
+  // per-element terms (a serial double sum of ~1e8 terms drifts by ~1e-12
+  // relative, more than the parity tolerance of an FP64 implementation).
+  long double final_energy_1 = 0.0L, final_energy_2 = 0.0L;
   const int size_idx_h1 = (int) base_size_h1b, size_idx_h2 = (int) base_size_h2b,
             size_idx_h3 = (int) base_size_h3b, size_idx_p4 = (int) base_size_p4b,
             size_idx_p5 = (int) base_size_p5b, size_idx_p6 = (int) base_size_p6b;
@@ -767,8 +771,8 @@ void task_kernel_cpu(const SOProblem& P, const Sizes& S, const size_t* t, double
               final_energy_1 += factor * t3d[idx_t3] * (t3d[idx_t3]) / inner_factor;
               final_energy_2 += factor * t3d[idx_t3] * (t3d[idx_t3] + t3s[idx_t3]) / inner_factor;
             }
-  out_e[0] = final_energy_1;
-  out_e[1] = final_energy_2;
+  out_e[0] = (double) final_energy_1;
+  out_e[1] = (double) final_energy_2;
 }
 
 SOProblem make_problem(int noab, int nvab, const int* tiles, const int* spins,
@@ -1045,7 +1049,7 @@ static int run_impl(SOProblem& P, int is_restricted, long nwhich, const long* wh
   build_tasks(P, is_restricted != 0, tasks, factors);
   Sizes       S = driver_sizes(P);
   TaskBuffers B;
-  double      energy_l[2] = {0.0, 0.0};
+  long double energy_l[2] = {0.0L, 0.0L};  // the reference's energy_l, carried in 80 bits
   long        n = nwhich > 0 ? nwhich : (long) factors.size();
   for(long q = 0; q < n; q++) {
     long k = nwhich > 0 ? which[q] : q;
@@ -1061,8 +1065,8 @@ static int run_impl(SOProblem& P, int is_restricted, long nwhich, const long* wh
       per_task[2 * q + 1] = e[1];
     }
   }
-  e_out[0] = energy_l[0];
-  e_out[1] = energy_l[1];
+  e_out[0] = (double) energy_l[0];
+  e_out[1] = (double) energy_l[1];
   return 0;
 }
 
