@@ -74,8 +74,15 @@ def cpu_baseline(tiling, s, per_task, budget_s):
         parts.append(pt[0])
         if spent > budget_s:
             break
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return dict(rate=flops / spent, spent=spent, pick=pick, flops=flops,
-                threads=O.num_threads(), partials=np.asarray(parts))
+                threads=O.num_threads(), partials=np.asarray(parts), model=model,
+                nproc=os.cpu_count())
 
 
 def main():
@@ -132,8 +139,10 @@ def main():
 
     def step():
         if which is None:
-            return ctx.run(precision=prec, schedule=sched, use_graph=graph, partials=False)
-        return ctx.run_tasks(which, precision=prec, schedule=sched, use_graph=graph)
+            return ctx.run(precision=prec, schedule=sched, use_graph=graph, partials=False,
+                           progress_ms=30000)
+        return ctx.run_tasks(which, precision=prec, schedule=sched, use_graph=graph,
+                             progress_ms=30000)
 
     for w in range(args.warmup):
         step()
@@ -184,8 +193,9 @@ def main():
             cw = np.sort(rng.choice(n_tasks, max(1, int(args.compare_frac * n_tasks)),
                                     replace=False))
         log(f"comparison on {len(cw)} tasks")
-        f = ctx.run_tasks(cw, precision=xt.PREC_FP64, schedule=sched, use_graph=graph)
-        m = ctx.run_tasks(cw, precision=prec, schedule=sched, use_graph=graph)
+        f = ctx.run_tasks(cw, precision=xt.PREC_FP64, schedule=sched, use_graph=graph,
+                          progress_ms=30000)
+        m = ctx.run_tasks(cw, precision=prec, schedule=sched, use_graph=graph, progress_ms=30000)
         d4, d5 = abs(m["e4"] - f["e4"]), abs(m["e5"] - f["e5"])
         line["fp64_compare"] = {
             "tasks": int(len(cw)), "full": bool(args.fp64_full),
@@ -208,7 +218,8 @@ def main():
             "value": total_flops / cb["rate"], "unit": "s (extrapolated time-to-E(T))",
             "cores": cb["threads"], "kind": "port",
             "sample": f"{len(cb['pick'])} tasks of this workload ({cb['flops']:.3e} flops) "
-                      f"timed in {cb['spent']:.1f}s = {cb['rate'] / 1e9:.2f} GFLOP/s, "
+                      f"timed in {cb['spent']:.1f}s = {cb['rate'] / 1e9:.2f} GFLOP/s on "
+                      f"{cb['threads']} OpenMP threads ({cb['model']}, nproc {cb['nproc']}), "
                       f"extrapolated by the analytic flop count"}
         chk = {}
         for name, p in (("fp64", xt.PREC_FP64), (args.precision, prec)):

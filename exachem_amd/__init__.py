@@ -40,7 +40,16 @@ def build(force: bool = False) -> str:
 class XtOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("schedule", C.c_int), ("use_graph", C.c_int),
                 ("record_task_time", C.c_int), ("workgroups", C.c_int),
-                ("reserved", C.c_int * 3)]
+                ("progress_ms", C.c_int), ("reserved", C.c_int * 2)]
+
+
+class LearnStats(C.Structure):
+    _fields_ = [("n_sample", C.c_long), ("final_loss", C.c_double),
+                ("fit_rmse_model", C.c_double), ("fit_rmse_flops", C.c_double),
+                ("calib_ms", C.c_double), ("train_ms", C.c_double)]
+
+
+MLP_IN, MLP_HIDDEN, MLP_PARAMS = 7, 64, 4737
 
 
 def lib():
@@ -73,12 +82,26 @@ def lib():
     L.xt_run_tasks.argtypes = [vp, C.POINTER(XtOpts), C.c_long, lp, dp, dp, C.POINTER(C.c_float)]
     L.xt_task_codes.argtypes = [vp, C.c_long, C.POINTER(C.c_uint64)]
     L.xt_prepare.argtypes = [vp, C.c_int]
+    fp = C.POINTER(C.c_float)
+    L.xt_mlp_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
+    L.xt_mlp_destroy.argtypes = [vp]
+    L.xt_mlp_get_params.argtypes = [vp, fp]
+    L.xt_mlp_set_params.argtypes = [vp, fp]
+    L.xt_mlp_loss_grad.argtypes = [vp, C.c_long, fp, fp, fp, fp]
+    L.xt_mlp_train.argtypes = [vp, C.c_long, fp, fp, C.c_int, C.c_float, fp]
+    L.xt_mlp_predict.argtypes = [vp, C.c_long, fp, fp]
+    L.xt_task_features.argtypes = [vp, C.c_int, fp]
+    L.xt_set_task_cost.argtypes = [vp, dp]
+    L.xt_learn_schedule.argtypes = [vp, vp, C.POINTER(XtOpts), C.c_long, C.c_int, C.c_int,
+                                    C.c_float, C.POINTER(LearnStats)]
     L.xt_fully_fused_ccsd_t_gpu.argtypes = (
         [vp, sz] + [sz] * 6 + [vp] * 6 + [ip] * 6 + [sz] * 8 + [vp] * 6 + [vp, vp])
     for name in ("xt_create", "xt_destroy", "xt_set_tiling", "xt_num_tasks", "xt_task_list",
                  "xt_flops", "xt_upload_closed_shell", "xt_synth_upload", "xt_synth_closed_shell",
                  "xt_run", "xt_last_kernel_ms", "xt_fully_fused_ccsd_t_gpu", "xt_run_tasks",
-                 "xt_task_codes", "xt_prepare"):
+                 "xt_task_codes", "xt_prepare", "xt_mlp_create", "xt_mlp_destroy",
+                 "xt_mlp_get_params", "xt_mlp_set_params", "xt_mlp_loss_grad", "xt_mlp_train",
+                 "xt_mlp_predict", "xt_task_features", "xt_set_task_cost", "xt_learn_schedule"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -98,7 +121,8 @@ def header_symbols(path: str = HEADER):
 
 
 from .triples import (  # noqa: E402
-    CCSDTriples, Tiling, default_opts, fully_fused_ccsd_t_gpu, synth_closed_shell, uneven_tiles)
+    CCSDTriples, CostModel, Tiling, default_opts, fully_fused_ccsd_t_gpu, synth_closed_shell,
+    uneven_tiles)
 
-__all__ = ["CCSDTriples", "Tiling", "default_opts", "fully_fused_ccsd_t_gpu", "synth_closed_shell",
-           "uneven_tiles", "lib", "build", "XtError", "XtOpts"]
+__all__ = ["CCSDTriples", "CostModel", "Tiling", "default_opts", "fully_fused_ccsd_t_gpu",
+           "synth_closed_shell", "uneven_tiles", "lib", "build", "XtError", "XtOpts", "LearnStats"]

@@ -115,6 +115,26 @@ int launch_zero_u64(unsigned long long* p, long n, void* stream);
 int build_split(const SpatialBufs& B, int O, int V, SplitBufs& S, void* stream);
 int launch_spatial_mixed(const MixedDev& M, const RunDev& R, int grid, void* stream);
 
+// Task cost model (xt_mlp.hip): 7 inputs (6 tile extents / 48, precision
+// flag) -> 64 -> 64 -> 1, FP32.
+constexpr int kMlpIn     = 7;
+constexpr int kMlpHidden = 64;
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden +
+                           kMlpHidden + kMlpHidden + 1;
+// Device buffers of one training batch of n samples.
+struct MlpWork {
+  float *P, *G, *M1, *M2;   // parameters, gradient, Adam moments   [kMlpParams]
+  float *X, *T, *Y;         // inputs [n][7], targets [n], outputs [n]
+  float *A1, *A2, *D1, *D2; // activations and error signals        [n][64]
+  float *DY, *SQ;           // [n]
+  float* loss;              // [1] mean squared error of the last pass
+  int*   step;              // [2] Adam step count, its value at the start of training
+};
+int mlp_forward(const float* P, const float* X, long n, float* Y, float* A1, float* A2,
+                void* stream);
+int mlp_loss_grad(const MlpWork& W, long n, void* stream);
+int mlp_train_step(const MlpWork& W, long n, float lr, float* loss_hist, void* stream);
+
 struct PackedArgs {
   int           dim[6];
   const double* d1_t2;

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -16,15 +17,22 @@
 
 #include "../../include/exachem_t.h"
 #include "xt_device.h"
+#include "xt_internal.h"
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+namespace xt {
+int set_error(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+}  // namespace xt
+
+namespace {
+
+inline int fail(int code, const std::string& msg) { return xt::set_error(code, msg); }
 
 #define XT_HIP(call)                                                                    \
   do {                                                                                  \
@@ -297,6 +305,7 @@ int current_device_is_gfx950(int dev) {
 struct xt_ctx {
   int         device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side   = nullptr;  // progress reads while the persistent kernel runs
   int         n_cu   = 0;
   Plan        plan;
   bool        have_plan = false, have_inputs = false;
@@ -412,6 +421,7 @@ int xt_create(int device, xt_ctx** out) {
   XT_HIP(hipGetDeviceProperties(&prop, device));
   c->n_cu = prop.multiProcessorCount;
   XT_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  XT_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
   XT_HIP(hipEventCreate(&c->ev0));
   XT_HIP(hipEventCreate(&c->ev1));
   *out = c;
@@ -431,6 +441,7 @@ int xt_destroy(xt_ctx* c) {
   if(c->ev0) (void) hipEventDestroy(c->ev0);
   if(c->ev1) (void) hipEventDestroy(c->ev1);
   if(c->stream) (void) hipStreamDestroy(c->stream);
+  if(c->side) (void) hipStreamDestroy(c->side);
   delete c;
   return XT_OK;
 }
@@ -649,6 +660,37 @@ static Queues build_queues(const Plan& P, const std::vector<int>& sel, int sched
   return Q;
 }
 
+// While the run is in flight, prints the share of queue tickets the
+// workgroups have taken (read from the device counters on a side stream).
+static int report_progress(xt_ctx* c, const xt_opts& o, const Queues& Q) {
+  if(o.progress_ms <= 0) return XT_OK;
+  long long total = 0;
+  for(long long t: Q.total) total += t;
+  unsigned long long ctr[xt::kQueues];
+  const auto         t0   = std::chrono::steady_clock::now();
+  auto               last = t0;
+  for(;;) {
+    // poll often so the caller regains control soon after the run drains
+    std::this_thread::sleep_for(std::chrono::milliseconds(2));
+    const hipError_t q = hipStreamQuery(c->stream);
+    if(q == hipSuccess) break;
+    if(q != hipErrorNotReady) return fail(XT_EHIP, std::string("progress: ") + hipGetErrorString(q));
+    const auto now = std::chrono::steady_clock::now();
+    if(now - last < std::chrono::milliseconds(o.progress_ms)) continue;
+    last = now;
+    XT_HIP(hipMemcpyAsync(ctr, c->d_qctr, sizeof(ctr), hipMemcpyDeviceToHost, c->side));
+    XT_HIP(hipStreamSynchronize(c->side));
+    long long taken = 0;
+    for(int q2 = 0; q2 < xt::kQueues; q2++)
+      taken += std::min<long long>((long long) ctr[q2], Q.total[q2]);
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "[xt] %.0f s: %lld of %lld tickets taken (%.1f%%)\n", s, taken, total,
+                 total ? 100.0 * (double) taken / (double) total : 100.0);
+    std::fflush(stderr);
+  }
+  return XT_OK;
+}
+
 // Launches the persistent kernel over the tasks `sel` (all if null), then the
 // per-task fixed-point -> FP64 conversion and the fixed-order sum.
 static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_sel) {
@@ -739,7 +781,10 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
     if(err) return fail(XT_EHIP, std::string("reduce: ") + hipGetErrorString((hipError_t) err));
     return XT_OK;
   };
-  if(!o.use_graph) return body(true);
+  if(!o.use_graph) {
+    int rc = body(true);
+    return rc ? rc : report_progress(c, o, Q);
+  }
   xt_ctx::GraphKey key;
   key.precision = o.precision;
   key.grid      = grid;
@@ -769,7 +814,7 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
   XT_HIP(hipEventRecord(c->ev0, st));
   XT_HIP(hipGraphLaunch(c->graph, st));
   XT_HIP(hipEventRecord(c->ev1, st));
-  return XT_OK;
+  return report_progress(c, o, Q);
 }
 
 // Summed workgroup time of each task (ms), from wall-clock ticks.
@@ -889,6 +934,117 @@ int xt_task_codes(xt_ctx* c, long task, uint64_t* codes) {
 int xt_last_kernel_ms(xt_ctx* c, double* ms) {
   if(!c || !ms) return fail(XT_EINVAL, "xt_last_kernel_ms: null argument");
   *ms = c->last_ms;
+  return XT_OK;
+}
+
+// ----------------------------------------------------- learned schedule
+int xt_task_features(xt_ctx* c, int precision, float* feats) {
+  if(!c || !feats) return fail(XT_EINVAL, "xt_task_features: null argument");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_task_features: no tiling");
+  if(precision != XT_PREC_FP64 && precision != XT_PREC_MIXED)
+    return fail(XT_EINVAL, "xt_task_features: unknown precision");
+  for(size_t k = 0; k < c->plan.dev.size(); k++) {
+    for(int p = 0; p < 6; p++) feats[7 * k + p] = (float) c->plan.dev[k].dim[p] / 48.f;
+    feats[7 * k + 6] = precision == XT_PREC_MIXED ? 1.f : 0.f;
+  }
+  return XT_OK;
+}
+
+int xt_set_task_cost(xt_ctx* c, const double* cost) {
+  if(!c || !cost) return fail(XT_EINVAL, "xt_set_task_cost: null argument");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_set_task_cost: no tiling");
+  const size_t n = c->plan.dev.size();
+  for(size_t k = 0; k < n; k++)
+    if(!(cost[k] > 0.0) || !std::isfinite(cost[k]))
+      return fail(XT_EINVAL, "xt_set_task_cost: costs must be positive and finite");
+  c->learned_cost.assign(cost, cost + n);
+  return XT_OK;
+}
+
+int xt_learn_schedule(xt_ctx* c, xt_mlp* m, const xt_opts* opts, long n_sample, int precision_mask,
+                      int steps, float lr, xt_learn_stats* stats) {
+  if(!c || !m || !opts || n_sample <= 0 || !(precision_mask & 3) || steps <= 0 || !(lr > 0.f))
+    return fail(XT_EINVAL, "xt_learn_schedule: bad argument");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_learn_schedule: no tiling");
+  const long n = (long) c->plan.dev.size();
+  if(n == 0) return fail(XT_ESTATE, "xt_learn_schedule: no tasks");
+  const auto t0 = std::chrono::steady_clock::now();
+  // tasks spread evenly over the flop-sorted list
+  std::vector<long> order(n);
+  std::iota(order.begin(), order.end(), 0L);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](long a, long b) { return c->plan.dev[a].flops < c->plan.dev[b].flops; });
+  const long        ns = std::min(n_sample, n);
+  std::vector<long> pick(ns);
+  for(long i = 0; i < ns; i++) pick[i] = order[ns == 1 ? n / 2 : (i * (n - 1)) / (ns - 1)];
+  std::sort(pick.begin(), pick.end());
+  pick.erase(std::unique(pick.begin(), pick.end()), pick.end());
+  // timing runs
+  std::vector<float>  x, t;
+  std::vector<double> lt, lf;  // log time, log flops of each sample
+  std::vector<float>  feats(7 * n);
+  for(int prec = 0; prec < 2; prec++) {
+    if(!(precision_mask & (1 << prec))) continue;
+    xt_opts o          = *opts;
+    o.precision        = prec == 0 ? XT_PREC_FP64 : XT_PREC_MIXED;
+    o.record_task_time = 1;
+    o.progress_ms      = 0;
+    std::vector<double> e45(2), part(2 * pick.size());
+    std::vector<float>  ms(pick.size());
+    int rc = xt_run_tasks(c, &o, (long) pick.size(), pick.data(), e45.data(), part.data(), ms.data());
+    if(rc) return rc;
+    if((rc = xt_task_features(c, o.precision, feats.data()))) return rc;
+    for(size_t i = 0; i < pick.size(); i++) {
+      if(!(ms[i] > 0.f)) continue;
+      const double fl = c->plan.dev[pick[i]].flops;
+      for(int j = 0; j < 7; j++) x.push_back(feats[7 * pick[i] + j]);
+      lt.push_back(std::log((double) ms[i]));
+      lf.push_back(std::log(fl));
+    }
+  }
+  const long   nt = (long) lt.size();
+  if(nt < 2) return fail(XT_ESTATE, "xt_learn_schedule: too few timed tasks");
+  const auto   t1 = std::chrono::steady_clock::now();
+  // standardised target: log time per flop
+  double mu = 0, sd = 0;
+  for(long i = 0; i < nt; i++) mu += lt[i] - lf[i];
+  mu /= nt;
+  for(long i = 0; i < nt; i++) sd += (lt[i] - lf[i] - mu) * (lt[i] - lf[i] - mu);
+  sd = std::sqrt(sd / nt);
+  if(!(sd > 1e-12)) sd = 1.0;
+  t.resize(nt);
+  for(long i = 0; i < nt; i++) t[i] = (float) ((lt[i] - lf[i] - mu) / sd);
+  std::vector<float> hist(steps);
+  int rc = xt_mlp_train(m, nt, x.data(), t.data(), steps, lr, hist.data());
+  if(rc) return rc;
+  std::vector<float> fit(nt);
+  if((rc = xt_mlp_predict(m, nt, x.data(), fit.data()))) return rc;
+  double se_m = 0, se_f = 0;
+  for(long i = 0; i < nt; i++) {
+    const double pm = lf[i] + mu + sd * fit[i], pf = lf[i] + mu;
+    se_m += (pm - lt[i]) * (pm - lt[i]);
+    se_f += (pf - lt[i]) * (pf - lt[i]);
+  }
+  // costs of every task at the scheduled precision
+  if((rc = xt_task_features(c, opts->precision, feats.data()))) return rc;
+  std::vector<float> y(n);
+  if((rc = xt_mlp_predict(m, n, feats.data(), y.data()))) return rc;
+  std::vector<double> cost(n);
+  for(long k = 0; k < n; k++) {
+    const double fl = c->plan.dev[k].flops;
+    const double cc = fl * std::exp(mu + sd * (double) y[k]);
+    cost[k]         = (std::isfinite(cc) && cc > 0.0) ? cc : fl;
+  }
+  c->learned_cost = cost;
+  const auto t2   = std::chrono::steady_clock::now();
+  if(stats) {
+    stats->n_sample       = (long) pick.size();
+    stats->final_loss     = hist.back();
+    stats->fit_rmse_model = std::sqrt(se_m / nt);
+    stats->fit_rmse_flops = std::sqrt(se_f / nt);
+    stats->calib_ms       = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    stats->train_ms       = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  }
   return XT_OK;
 }
 

@@ -15,7 +15,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import PREC_FP64, SCHED_DRIVER, XtOpts, check, lib
+from . import MLP_IN, MLP_PARAMS, PREC_FP64, PREC_MIXED, SCHED_DRIVER, LearnStats, XtOpts, check, lib
 
 
 def _dp(a):
@@ -24,6 +24,78 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+class CostModel:
+    """Task cost model (7 -> 64 -> 64 -> 1 FP32 perceptron) whose training and
+    inference run as HIP kernels on `device`.  Parameters: flat float32 array
+    W1[7][64] b1[64] W2[64][64] b2[64] w3[64] b3."""
+
+    def __init__(self, device: int = 0, seed: int = 1):
+        self._m = C.c_void_p()
+        check(lib().xt_mlp_create(device, seed, C.byref(self._m)))
+
+    def close(self):
+        if self._m:
+            lib().xt_mlp_destroy(self._m)
+            self._m = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._m
+
+    def get_params(self):
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        check(lib().xt_mlp_get_params(self._m, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = _f32(p, (MLP_PARAMS,))
+        check(lib().xt_mlp_set_params(self._m, _fp(p)))
+
+    def loss_grad(self, x, t):
+        x = _f32(x)
+        t = _f32(t)
+        n = len(t)
+        if x.shape != (n, MLP_IN):
+            raise ValueError("x must be (n, 7)")
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        check(lib().xt_mlp_loss_grad(self._m, n, _fp(x), _fp(t), _fp(loss), _fp(g)))
+        return float(loss[0]), g
+
+    def train(self, x, t, steps, lr):
+        x = _f32(x)
+        t = _f32(t)
+        n = len(t)
+        if x.shape != (n, MLP_IN):
+            raise ValueError("x must be (n, 7)")
+        hist = np.zeros(max(steps, 1), dtype=np.float32)
+        check(lib().xt_mlp_train(self._m, n, _fp(x), _fp(t), steps, lr, _fp(hist)))
+        return hist[:steps]
+
+    def predict(self, x):
+        x = _f32(x)
+        y = np.zeros(len(x), dtype=np.float32)
+        check(lib().xt_mlp_predict(self._m, len(x), _fp(x), _fp(y)))
+        return y
 
 
 def uniform_tiles(n: int, ts: int):
@@ -154,10 +226,11 @@ class CCSDTriples:
         check(lib().xt_prepare(self._ctx, precision))
 
     def run(self, precision=PREC_FP64, schedule=SCHED_DRIVER, use_graph=0, record_task_time=0,
-            workgroups=0, partials=True):
+            workgroups=0, partials=True, progress_ms=0):
         n = self.num_tasks()
         o = default_opts(precision=precision, schedule=schedule, use_graph=use_graph,
-                         record_task_time=record_task_time, workgroups=workgroups)
+                         record_task_time=record_task_time, workgroups=workgroups,
+                         progress_ms=progress_ms)
         e = np.zeros(2)
         tp = np.zeros(2 * max(n, 1)) if partials else None
         tm = np.zeros(max(n, 1), dtype=np.float32) if record_task_time else None
@@ -170,12 +243,13 @@ class CCSDTriples:
                     task_ms=tm[:n] if tm is not None else None, kernel_ms=ms.value)
 
     def run_tasks(self, which, precision=PREC_FP64, schedule=SCHED_DRIVER, use_graph=0,
-                  record_task_time=0, workgroups=0):
+                  record_task_time=0, workgroups=0, progress_ms=0):
         """Runs the listed tasks only; partials come back in listed order."""
         which = np.ascontiguousarray(which, dtype=np.int64)
         n = len(which)
         o = default_opts(precision=precision, schedule=schedule, use_graph=use_graph,
-                         record_task_time=record_task_time, workgroups=workgroups)
+                         record_task_time=record_task_time, workgroups=workgroups,
+                         progress_ms=progress_ms)
         e = np.zeros(2)
         tp = np.zeros(2 * n)
         tm = np.zeros(n, dtype=np.float32)
@@ -186,6 +260,28 @@ class CCSDTriples:
         check(lib().xt_last_kernel_ms(self._ctx, C.byref(ms)))
         return dict(e4=float(e[0]), e5=float(e[1]), task_partials=tp.reshape(n, 2),
                     task_ms=tm if record_task_time else None, kernel_ms=ms.value)
+
+    def task_features(self, precision=PREC_FP64):
+        """Cost-model inputs of every task: (n, 7) float32."""
+        f = np.zeros((max(self.num_tasks(), 1), MLP_IN), dtype=np.float32)
+        check(lib().xt_task_features(self._ctx, precision, _fp(f)))
+        return f[:self.num_tasks()]
+
+    def set_task_cost(self, cost):
+        """Per-task costs for SCHED_LEARNED."""
+        cost = np.ascontiguousarray(cost, dtype=np.float64)
+        check(lib().xt_set_task_cost(self._ctx, _dp(cost)))
+
+    def learn_schedule(self, model: CostModel, precision=PREC_MIXED, n_sample=256,
+                       precisions=(PREC_FP64, PREC_MIXED), steps=1500, lr=3e-3, workgroups=0):
+        """Times n_sample tasks per precision, trains `model` on the GPU and
+        installs its predicted costs for SCHED_LEARNED.  Returns the stats."""
+        o = default_opts(precision=precision, workgroups=workgroups)
+        mask = sum(1 << (0 if p == PREC_FP64 else 1) for p in precisions)
+        st = LearnStats()
+        check(lib().xt_learn_schedule(self._ctx, model.handle, C.byref(o), n_sample, mask, steps,
+                                      lr, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in LearnStats._fields_}
 
     def task_codes(self, task):
         """(d1code, d2code, s1code) bit fields of one task."""

@@ -149,7 +149,9 @@ typedef struct xt_opts {
   int   use_graph;   /* 1: capture the launch sequence in a hipGraph and replay it   */
   int   record_task_time; /* 1: record per-task device timestamps                    */
   int   workgroups;  /* persistent grid size (0 = library default)                   */
-  int   reserved[3];
+  int   progress_ms; /* > 0: while the kernel runs, print the fraction of work-queue
+                        tickets taken to stderr every progress_ms milliseconds       */
+  int   reserved[2];
 } xt_opts;
 
 /* Default options (FP64, driver order, graph on, timing off). */
@@ -184,6 +186,64 @@ int xt_task_codes(xt_ctx* ctx, long task, uint64_t* codes);
 
 /* Device time of the last xt_run's main kernel (ms), from HIP events. */
 int xt_last_kernel_ms(xt_ctx* ctx, double* ms);
+
+/* ------------------------------------------------------------------------
+ * Learned task-cost model for the work-queue scheduler (no counterpart in
+ * the reference, whose tasks are handed out in driver order by NXTVAL,
+ * ccsd_t_fused_driver.hpp:381).
+ *
+ * A 7 -> 64 -> 64 -> 1 FP32 perceptron (ReLU hidden layers) whose training
+ * (forward, back-propagation, Adam) and inference run as HIP kernels on the
+ * model's device.  Inputs per task (xt_task_features): the six tile extents
+ * / 48 and the precision flag (0 FP64, 1 mixed).  The scheduler uses it to
+ * predict log(task time / analytic flops); a task's cost is its flop count
+ * times the predicted time per flop.
+ * ------------------------------------------------------------------------ */
+#define XT_MLP_IN 7
+#define XT_MLP_HIDDEN 64
+#define XT_MLP_PARAMS 4737 /* W1[7][64] b1[64] W2[64][64] b2[64] w3[64] b3 */
+
+typedef struct xt_mlp xt_mlp;
+
+/* He-initialised parameters from a seeded generator (biases zero). */
+int xt_mlp_create(int device, uint64_t seed, xt_mlp** out);
+int xt_mlp_destroy(xt_mlp* m);
+/* Parameters in the layout above; set also clears the Adam state. */
+int xt_mlp_get_params(xt_mlp* m, float* params);
+int xt_mlp_set_params(xt_mlp* m, const float* params);
+/* Mean squared error over the batch x[n][7] -> t[n] and its gradient with
+ * respect to every parameter (grad[XT_MLP_PARAMS]); parameters unchanged. */
+int xt_mlp_loss_grad(xt_mlp* m, long n, const float* x, const float* t, float* loss, float* grad);
+/* `steps` full-batch Adam steps (beta 0.9/0.999, eps 1e-8) at rate lr,
+ * replayed from one captured graph; loss_hist[steps] (optional) receives the
+ * loss before each step.  Deterministic. */
+int xt_mlp_train(xt_mlp* m, long n, const float* x, const float* t, int steps, float lr,
+                 float* loss_hist);
+/* y[n] = model(x[n][7]). */
+int xt_mlp_predict(xt_mlp* m, long n, const float* x, float* y);
+
+/* Model inputs of every task of the tiling at a precision: feats[n_tasks][7]. */
+int xt_task_features(xt_ctx* ctx, int precision, float* feats);
+/* Per-task costs used by XT_SCHED_LEARNED (any positive unit). */
+int xt_set_task_cost(xt_ctx* ctx, const double* cost);
+
+typedef struct xt_learn_stats {
+  long   n_sample;         /* tasks timed per precision                        */
+  double final_loss;       /* training loss (standardised log time per flop)   */
+  double fit_rmse_model;   /* RMS error of log(time) on the timed tasks: model  */
+  double fit_rmse_flops;   /* same for time = flops / (best single rate)       */
+  double calib_ms;         /* wall time of the timing runs                      */
+  double train_ms;         /* wall time of training and prediction              */
+} xt_learn_stats;
+
+/* Learned schedule: times n_sample tasks (spread over the flop range) at
+ * each precision in precision_mask (bit 0 FP64, bit 1 mixed), trains the
+ * model on log(time/flops) (standardised), predicts every task at
+ * opts->precision and installs flops * predicted time per flop as the
+ * XT_SCHED_LEARNED costs.  The analytic flop count stays the fallback: a
+ * non-finite or non-positive prediction leaves that task at its flop cost. */
+int xt_learn_schedule(xt_ctx* ctx, xt_mlp* m, const xt_opts* opts, long n_sample,
+                      int precision_mask, int steps, float lr, xt_learn_stats* stats);
 
 #ifdef __cplusplus
 }

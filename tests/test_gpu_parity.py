@@ -8,10 +8,14 @@ its values are the correctly rounded sums; the stated input distributions
 resolves no better than ~1e-16 |E|, so the relative term covers what an
 absolute 1e-10 Eh cannot.
 
-Tolerance (mixed): |dE| <= 1e-7 |E4| for E[4], E[5] and each task.  The
-contractions run as hi*hi + hi*lo + lo*hi FP16 products (~22 significant
-bits) with FP32 accumulation; the measured error is ~1e-9 relative, and
-dropping any one of the three products moves E by ~1e-4 relative."""
+Tolerance (mixed): |dE| <= 1e-7 |E4| for the totals E[4], E[5] and
+<= 1e-6 |E4| for each task.  The contractions run as hi*hi + hi*lo + lo*hi
+FP16 products (~22 significant bits, the dropped lo*lo term ~2^-22 of a
+product) with FP32 accumulation, so a t3 element carries ~1e-6 of the sum of
+the magnitudes of its terms; a task's energy averages that over its
+elements (measured up to ~1e-7 on tasks of a few hundred elements) and the
+totals over every task (~1e-9).  Dropping any one of the three products moves
+E by ~1e-4 relative."""
 import ctypes as C
 import json
 import os
@@ -23,15 +27,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 pytestmark = pytest.mark.gpu
 
 ABS, REL = 1e-10, 2e-14
-MIX_REL = 1e-7
+MIX_REL, MIX_TASK_REL = 1e-7, 1e-6
 
 
 def close(a, b):
     return abs(a - b) <= ABS + REL * abs(b)
 
 
-def close_mixed(a, b, scale):
-    return abs(a - b) <= MIX_REL * abs(scale)
+def close_mixed(a, b, scale, rel=MIX_REL):
+    return abs(a - b) <= rel * abs(scale)
 
 
 def small_cases():
@@ -156,8 +160,8 @@ def test_mixed_matches_oracle(xt, name):
     ref = np.asarray(case["task_partials"])
     got = r["task_partials"]
     bad = [(k, got[k].tolist(), ref[k].tolist()) for k in range(len(ref))
-           if not (close_mixed(got[k, 0], ref[k, 0], ref[k, 0]) and
-                   close_mixed(got[k, 1], ref[k, 1], ref[k, 0]))]
+           if not (close_mixed(got[k, 0], ref[k, 0], ref[k, 0], MIX_TASK_REL) and
+                   close_mixed(got[k, 1], ref[k, 1], ref[k, 0], MIX_TASK_REL))]
     assert not bad, bad[:5]
     ctx.close()
 
@@ -190,8 +194,8 @@ def test_config1_sampled_tasks_mixed(xt):
         got = r["task_partials"][i]
         print(f"[gpu] c1 mixed task {k}: rel dE4 {(got[0] - ref[i, 0]) / ref[i, 0]:.2e}",
               flush=True)
-        assert close_mixed(got[0], ref[i, 0], ref[i, 0]), (k, got, ref[i])
-        assert close_mixed(got[1], ref[i, 1], ref[i, 0]), (k, got, ref[i])
+        assert close_mixed(got[0], ref[i, 0], ref[i, 0], MIX_TASK_REL), (k, got, ref[i])
+        assert close_mixed(got[1], ref[i, 1], ref[i, 0], MIX_TASK_REL), (k, got, ref[i])
     ctx.close()
 
 
