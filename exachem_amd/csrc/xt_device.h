@@ -70,6 +70,7 @@ struct SplitBufs {
   unsigned char* v[kOperands];   // [rows][Kp16/8][2][8] halves
   short*         e[kOperands];   // [rows]
   long           rows[kOperands];
+  unsigned char* zero;           // zero row, 4 * max(Kp16) bytes
 };
 struct MixedDev {
   int                  O, V, Ko, Kv;  // Ko = roundup(O,16), Kv = roundup(V,16)
@@ -79,6 +80,7 @@ struct MixedDev {
   const double*        vvoo;
   const unsigned char* v[kOperands];
   const short*         e[kOperands];
+  const unsigned char* zero;  // zero row (4 * max(Ko, Kv) bytes): rows outside a tile read it
 };
 
 // Work queues: one per XCD.  Queue q holds the tasks qtask[qbeg[q]..qbeg[q+1])
@@ -98,6 +100,7 @@ struct RunDev {
   unsigned long long* acc;     // [4*n_tasks] (E4 lo, E4 hi, E5 lo, E5 hi): 128-bit
                                // two's-complement fixed point, scale 2^64 per Eh
   unsigned long long* cost;    // [n_tasks] summed workgroup wall-clock ticks (optional)
+  unsigned long long* prog;    // [kQueues] host-mapped mirror of the ticket counters (optional)
   int                 n_tasks;
 };
 

@@ -305,7 +305,10 @@ int current_device_is_gfx950(int dev) {
 struct xt_ctx {
   int         device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t side   = nullptr;  // progress reads while the persistent kernel runs
+  // host-mapped mirror of the queue ticket counters: the kernel stores the
+  // tickets it takes, the host reads them while the persistent kernel runs
+  unsigned long long* h_prog = nullptr;
+  unsigned long long* d_prog = nullptr;
   int         n_cu   = 0;
   Plan        plan;
   bool        have_plan = false, have_inputs = false;
@@ -368,6 +371,8 @@ struct xt_ctx {
       split.v[j] = nullptr;
       split.e[j] = nullptr;
     }
+    if(split.zero) (void) hipFree(split.zero);
+    split.zero = nullptr;
     have_split = false;
   }
   void free_plan_dev() {
@@ -421,7 +426,9 @@ int xt_create(int device, xt_ctx** out) {
   XT_HIP(hipGetDeviceProperties(&prop, device));
   c->n_cu = prop.multiProcessorCount;
   XT_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  XT_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  XT_HIP(hipHostMalloc((void**) &c->h_prog, sizeof(unsigned long long) * xt::kQueues,
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  XT_HIP(hipHostGetDevicePointer((void**) &c->d_prog, c->h_prog, 0));
   XT_HIP(hipEventCreate(&c->ev0));
   XT_HIP(hipEventCreate(&c->ev1));
   *out = c;
@@ -441,7 +448,7 @@ int xt_destroy(xt_ctx* c) {
   if(c->ev0) (void) hipEventDestroy(c->ev0);
   if(c->ev1) (void) hipEventDestroy(c->ev1);
   if(c->stream) (void) hipStreamDestroy(c->stream);
-  if(c->side) (void) hipStreamDestroy(c->side);
+  if(c->h_prog) (void) hipHostFree(c->h_prog);
   delete c;
   return XT_OK;
 }
@@ -661,12 +668,12 @@ static Queues build_queues(const Plan& P, const std::vector<int>& sel, int sched
 }
 
 // While the run is in flight, prints the share of queue tickets the
-// workgroups have taken (read from the device counters on a side stream).
+// workgroups have taken (from the host-mapped mirror the kernels write; a
+// copy on another stream would wait for the persistent kernel's CUs).
 static int report_progress(xt_ctx* c, const xt_opts& o, const Queues& Q) {
   if(o.progress_ms <= 0) return XT_OK;
   long long total = 0;
   for(long long t: Q.total) total += t;
-  unsigned long long ctr[xt::kQueues];
   const auto         t0   = std::chrono::steady_clock::now();
   auto               last = t0;
   for(;;) {
@@ -678,11 +685,11 @@ static int report_progress(xt_ctx* c, const xt_opts& o, const Queues& Q) {
     const auto now = std::chrono::steady_clock::now();
     if(now - last < std::chrono::milliseconds(o.progress_ms)) continue;
     last = now;
-    XT_HIP(hipMemcpyAsync(ctr, c->d_qctr, sizeof(ctr), hipMemcpyDeviceToHost, c->side));
-    XT_HIP(hipStreamSynchronize(c->side));
     long long taken = 0;
-    for(int q2 = 0; q2 < xt::kQueues; q2++)
-      taken += std::min<long long>((long long) ctr[q2], Q.total[q2]);
+    for(int q2 = 0; q2 < xt::kQueues; q2++) {
+      const unsigned long long v = __atomic_load_n(&c->h_prog[q2], __ATOMIC_RELAXED);
+      taken += std::min<long long>((long long) v, Q.total[q2]);
+    }
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::fprintf(stderr, "[xt] %.0f s: %lld of %lld tickets taken (%.1f%%)\n", s, taken, total,
                  total ? 100.0 * (double) taken / (double) total : 100.0);
@@ -744,6 +751,8 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
   R.qctr    = c->d_qctr;
   R.acc     = c->d_acc;
   R.cost    = o.record_task_time ? c->d_cost : nullptr;
+  R.prog    = c->d_prog;
+  for(int q = 0; q < xt::kQueues; q++) __atomic_store_n(&c->h_prog[q], 0ull, __ATOMIC_RELAXED);
   R.n_tasks = (int) nt;
   long long tickets = 0;
   for(long long t: Q.total) tickets += t;
@@ -763,6 +772,7 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
       M.v[j] = c->split.v[j];
       M.e[j] = c->split.e[j];
     }
+    M.zero = c->split.zero;
     grid = o.workgroups > 0 ? o.workgroups : c->n_cu;
     grid = (int) std::max<long long>(1, std::min<long long>(grid, tickets));
   }

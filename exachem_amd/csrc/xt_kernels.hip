@@ -378,6 +378,8 @@ __global__ void __launch_bounds__(NT) k_spatial_fp64(SpatialDev S, RunDev R) {
       __syncthreads();  // previous sub-tile finished with LDS and s_task
       if(tid == 0) {
         const unsigned long long t = atomicAdd(&R.qctr[q], 1ull);
+        // progress mirror in host memory (tickets taken so far, approximate)
+        if(R.prog) __hip_atomic_store(&R.prog[q], t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         if((long long) t >= R.qtotal[q]) s_task = -1;
         else {
           int lo = R.qbeg[q], hi = R.qbeg[q + 1] - 1;  // last entry with qtick <= t

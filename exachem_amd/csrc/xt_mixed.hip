@@ -12,11 +12,16 @@
 //     hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_f16 with FP32 accumulation
 //     (~22-bit products); the row scales and the spin-orbital sign are undone
 //     when a K part ends and the result is added into the FP32 box;
-//   * K chunks of 16 stream through LDS with a register prefetch that runs
-//     across part and bipartition boundaries; MFMA fragments are double
-//     buffered in registers so each chunk's MFMAs overlap the next chunk's
-//     LDS reads;
-//   * s1, the denominators and the energy partials are FP64, reduced and
+//   * wave w owns M rows 64w..64w+63: their A fragments are loaded from HBM/L2
+//     straight into registers in the MFMA operand layout (no LDS round trip);
+//     the B panel (shared by the four waves) goes through a double-buffered
+//     LDS stage.  A register ring keeps PF chunks of loads in flight across
+//     part and bipartition boundaries, and one barrier per K chunk separates
+//     the B stage of chunk c+2 and the fragment reads of chunk c+1 from the
+//     MFMAs of chunk c;
+//   * energy: one FP64 reciprocal of the denominator per element (from
+//     separable hole and particle sums), E4 in FP64, then t3/D is kept in
+//     place for the s1 sweeps (FP32 products, FP64 sums); task partials are
 //     accumulated exactly like the FP64 kernel.
 #include <hip/hip_runtime.h>
 
@@ -28,22 +33,34 @@ namespace {
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float    f16v __attribute__((ext_vector_type(16)));
 
-constexpr int NT   = 256;
-constexpr int MM   = 256;  // bipartition rows   (hs, pA, pB)
-constexpr int NN   = 128;  // bipartition columns (hA, hB, ps)
-constexpr int KC   = 16;   // K per chunk
-constexpr int ROWB = 80;   // staged row stride (bytes): hi[16] lo[16], then the row's
-                           // FP32 scale for parts of even / odd ordinal at bytes 64 / 68
-constexpr int BOX  = 32768;
-constexpr int PF   = 2;    // chunks in flight per thread (plus one in registers)
+constexpr int NT  = 256;
+constexpr int MM  = 256;  // bipartition rows   (hs, pA, pB)
+constexpr int NN  = 128;  // bipartition columns (hA, hB, ps)
+constexpr int KC  = 16;   // K per chunk (one MFMA K step)
+constexpr int BOX = 32768;
+constexpr int PF  = 4;    // ring depth: chunks whose loads are in flight per thread
+constexpr int S1P = 3;    // s1 slots staged per sweep
 
 struct __align__(16) MSmem {
-  float         t3[BOX];
-  unsigned char a[MM * ROWB];
-  unsigned char b[NN * ROWB];
-  double        eps[36];  // h1,h2,h3 (4 each) at 0,4,8; p4,p5,p6 (8 each) at 12,20,28
-  double        red[8];
-  int           sub[6], dim[6], off[6];
+  float t3[BOX];
+  union {
+    struct {
+      uint4 hi[2][2][NN];  // [buffer][k half][column]: FP16 hi of 8 k
+      uint4 lo[2][2][NN];  // FP16 lo of the same 8 k
+      float sb[3][NN];     // column scales of parts (ordinal mod 3)
+    } g;
+    struct {
+      double dh[64];   // e_h1 + e_h2 + e_h3 over the box's holes   (index h1 | h2<<2 | h3<<4)
+      double dp[512];  // e_p4 + e_p5 + e_p6 over the box's particles (p4 | p5<<3 | p6<<6)
+    } d;
+    struct {
+      float t1[S1P][32];    // [slot][pa*4 + hi]
+      float v1[S1P][1024];  // [slot][hk + 4 hj + 16 pc + 128 pb]
+    } s;
+  } u;
+  double eps[36];  // h1,h2,h3 (4 each) at 0,4,8; p4,p5,p6 (8 each) at 12,20,28
+  double red[8];
+  int    sub[6], dim[6], off[6];
 };
 
 // bit position of each t3 index in the box layout (2 bits per hole, 3 per particle)
@@ -76,19 +93,24 @@ __device__ __forceinline__ void fx_add(unsigned long long* acc, double p) {
   atomicAdd(&acc[1], hi + carry);
 }
 
-// One K part of a bipartition, as seen by one thread: its staged A row
-// (m = tid) and half of B row (n = tid/2), their pre-split data and scales.
+__device__ __forceinline__ h8 as_h8(uint4 u) { return __builtin_bit_cast(h8, u); }
+
+// One K part of a bipartition as seen by one thread: the two A rows of its
+// lane (M rows 64w + 32i + lane%32, k half lane/32) and its B column half
+// (column tid/2, k half tid%2), with their scales.
 struct Part {
-  const unsigned char* pa;  // A row data (null: zero row)
-  const unsigned char* pb;  // B row data
-  float                sa, sb;
+  const unsigned char* pa[2];  // null: zero row
+  const unsigned char* pb;
+  float                sa[2], sb;
   int                  nch;
 };
 
-struct Ring {
-  uint4 a[4];  // 64 B: k0..7 [hi8 lo8], k8..15 [hi8 lo8]
-  uint4 b[2];  // 32 B: this thread's 8-k half [hi8 lo8]
-  float sa, sb;
+// One ring slot: the loads of one K chunk plus what its consumers need.
+struct Slot {
+  uint4 a[2][2];  // [row tile i][hi, lo]: 8 k of this lane's k half
+  uint4 b[2];     // hi, lo of the staged column half
+  float sa[2], sb;
+  int   tag;      // 2 * part ordinal + (first chunk of its part)
 };
 
 // Part gp = 2*g + (d2 ? 1 : 0) of bipartition g for this thread.
@@ -98,157 +120,153 @@ __device__ Part part_setup(const MSmem& sm, const MixedDev& M, int gp, unsigned 
   int       hA, hB, pA, pB;
   pair_of(hs, hA, hB);
   pair_of(ps, pA, pB);
-  const int tid = threadIdx.x;
-  const int m = tid, n = tid >> 1, half = tid & 1;
-  const int gHS = 4 * sm.sub[hs] + (m & 3), gPA = 8 * sm.sub[3 + pA] + ((m >> 2) & 7),
-            gPB = 8 * sm.sub[3 + pB] + (m >> 5);
-  const int gHA = 4 * sm.sub[hA] + (n & 3), gHB = 4 * sm.sub[hB] + ((n >> 2) & 3),
-            gPS = 8 * sm.sub[3 + ps] + (n >> 4);
-  const bool mok = gHS < sm.dim[hs] && gPA < sm.dim[3 + pA] && gPB < sm.dim[3 + pB];
-  const bool nok = gHA < sm.dim[hA] && gHB < sm.dim[hB] && gPS < sm.dim[3 + ps];
-  const long O = M.O, V = M.V;
-  const long HS = sm.off[hs] + (mok ? gHS : 0), PA = sm.off[3 + pA] + (mok ? gPA : 0),
-             PB = sm.off[3 + pB] + (mok ? gPB : 0);
-  const long HA = sm.off[hA] + (nok ? gHA : 0), HB = sm.off[hB] + (nok ? gHB : 0),
-             PS = sm.off[3 + ps] + (nok ? gPS : 0);
+  const int  tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int  O = M.O, V = M.V;      // row indices stay below 2^31 (V*V*O rows at most)
+  const long K = d2 ? M.Kv : M.Ko;  // padded row length (elements)
   const float sg = (code & 2) ? -1.f : 1.f;
   // row index and operand array of c1*x - c2*y (both -> antisymmetrised array)
-  auto choose = [](unsigned c, unsigned bx, unsigned by, int both, int single, long rx, long ry,
-                   int& arr, long& row, float& coef) {
+  auto choose = [](unsigned c, unsigned bx, unsigned by, int both, int single, int rx, int ry,
+                   int& arr, int& row, float& coef) {
     const bool hx = (c & bx) != 0, hy = (c & by) != 0;
     arr  = hx && hy ? both : single;
     row  = hx ? rx : ry;
     coef = (hx || hy) ? (hx ? 1.f : -1.f) : 0.f;
   };
-  int   aa, ab;
-  long  ra, rb;
-  float ca, cb;
-  if(!d2) {
-    choose(code, 4, 8, 1, 0, (PA * V + PB) * O + HS, (PB * V + PA) * O + HS, aa, ra, ca);
-    choose(code, 16, 32, 3, 2, (HB * V + PS) * O + HA, (HA * V + PS) * O + HB, ab, rb, cb);
-  }
-  else {
-    choose(code, 4, 8, 5, 4, (PB * V + PA) * O + HS, (PA * V + PB) * O + HS, aa, ra, ca);
-    choose(code, 16, 32, 7, 6, (HA * O + HB) * V + PS, (HB * O + HA) * V + PS, ab, rb, cb);
-  }
-  const long K = d2 ? M.Kv : M.Ko;  // padded row length (elements)
-  Part       P;
+  Part P;
   P.nch = (int) (K / KC);
-  P.pa  = (mok && ca != 0.f) ? M.v[aa] + ra * K * 4 : nullptr;
-  P.pb  = (nok && cb != 0.f) ? M.v[ab] + rb * K * 4 + 32 * half : nullptr;
-  P.sa  = P.pa ? sg * ca * ldexpf(1.f, -(int) M.e[aa][ra]) : 0.f;
-  P.sb  = P.pb ? cb * ldexpf(1.f, -(int) M.e[ab][rb]) : 0.f;
+#pragma unroll
+  for(int i = 0; i < 2; i++) {
+    const int  m   = 64 * wave + 32 * i + (lane & 31);
+    const int  gHS = 4 * sm.sub[hs] + (m & 3), gPA = 8 * sm.sub[3 + pA] + ((m >> 2) & 7),
+               gPB = 8 * sm.sub[3 + pB] + (m >> 5);
+    const bool mok = gHS < sm.dim[hs] && gPA < sm.dim[3 + pA] && gPB < sm.dim[3 + pB];
+    const int HS = sm.off[hs] + (mok ? gHS : 0), PA = sm.off[3 + pA] + (mok ? gPA : 0),
+              PB = sm.off[3 + pB] + (mok ? gPB : 0);
+    int   aa, ra;
+    float ca;
+    if(!d2) choose(code, 4, 8, 1, 0, (PA * V + PB) * O + HS, (PB * V + PA) * O + HS, aa, ra, ca);
+    else choose(code, 4, 8, 5, 4, (PB * V + PA) * O + HS, (PA * V + PB) * O + HS, aa, ra, ca);
+    P.pa[i] = (mok && ca != 0.f) ? M.v[aa] + (long) ra * K * 4 + 32 * (lane >> 5) : nullptr;
+    P.sa[i] = P.pa[i] ? sg * ca * ldexpf(1.f, -(int) M.e[aa][ra]) : 0.f;
+  }
+  const int  n = tid >> 1, half = tid & 1;
+  const int  gHA = 4 * sm.sub[hA] + (n & 3), gHB = 4 * sm.sub[hB] + ((n >> 2) & 3),
+             gPS = 8 * sm.sub[3 + ps] + (n >> 4);
+  const bool nok = gHA < sm.dim[hA] && gHB < sm.dim[hB] && gPS < sm.dim[3 + ps];
+  const int HA = sm.off[hA] + (nok ? gHA : 0), HB = sm.off[hB] + (nok ? gHB : 0),
+            PS = sm.off[3 + ps] + (nok ? gPS : 0);
+  int   ab, rb;
+  float cb;
+  if(!d2) choose(code, 16, 32, 3, 2, (HB * V + PS) * O + HA, (HA * V + PS) * O + HB, ab, rb, cb);
+  else choose(code, 16, 32, 7, 6, (HA * O + HB) * V + PS, (HB * O + HA) * V + PS, ab, rb, cb);
+  P.pb = (nok && cb != 0.f) ? M.v[ab] + (long) rb * K * 4 + 32 * half : nullptr;
+  P.sb = P.pb ? cb * ldexpf(1.f, -(int) M.e[ab][rb]) : 0.f;
   return P;
 }
 
-__device__ __forceinline__ void fetch(const Part& P, int ch, Ring& r) {
+__device__ __forceinline__ void fetch(const Part& P, int ch, int tag, Slot& s) {
   const uint4 z = make_uint4(0, 0, 0, 0);
-  if(P.pa) {
-    const uint4* q = reinterpret_cast<const uint4*>(P.pa + (long) ch * 64);
-    r.a[0] = q[0]; r.a[1] = q[1]; r.a[2] = q[2]; r.a[3] = q[3];
-  }
-  else {
-    r.a[0] = z; r.a[1] = z; r.a[2] = z; r.a[3] = z;
+#pragma unroll
+  for(int i = 0; i < 2; i++) {
+    if(P.pa[i]) {
+      const uint4* q = reinterpret_cast<const uint4*>(P.pa[i] + (long) ch * 64);
+      s.a[i][0]      = q[0];
+      s.a[i][1]      = q[1];
+    }
+    else {
+      s.a[i][0] = z;
+      s.a[i][1] = z;
+    }
   }
   if(P.pb) {
     const uint4* q = reinterpret_cast<const uint4*>(P.pb + (long) ch * 64);
-    r.b[0] = q[0]; r.b[1] = q[1];
+    s.b[0]         = q[0];
+    s.b[1]         = q[1];
   }
   else {
-    r.b[0] = z; r.b[1] = z;
+    s.b[0] = z;
+    s.b[1] = z;
   }
-  r.sa = P.sa;
-  r.sb = P.sb;
+  s.sa[0] = P.sa[0];
+  s.sa[1] = P.sa[1];
+  s.sb    = P.sb;
+  s.tag   = tag;
 }
 
-__device__ __forceinline__ float& row_scale(unsigned char* row, int parity) {
-  return *reinterpret_cast<float*>(row + 64 + 4 * parity);
+// Writes a chunk's B column halves into LDS buffer `buf`; the first chunk of
+// a part also publishes the part's column scales.
+__device__ __forceinline__ void stage(MSmem& sm, const Slot& s, int buf) {
+  const int tid = threadIdx.x, n = tid >> 1, half = tid & 1;
+  sm.u.g.hi[buf][half][n] = s.b[0];
+  sm.u.g.lo[buf][half][n] = s.b[1];
+  if((s.tag & 1) && !half) sm.u.g.sb[(s.tag >> 1) % 3][n] = s.sb;
 }
 
-// Stages one chunk: A row m=tid as hi[k0..15] | lo[k0..15]; B row n=tid/2
-// half.  The first chunk of a part also stores the row scales of that part in
-// the slot of its ordinal's parity.
-__device__ __forceinline__ void stage(MSmem& sm, const Ring& r, bool first, int parity) {
-  const int tid = threadIdx.x;
-  uint4*    ar  = reinterpret_cast<uint4*>(sm.a + tid * ROWB);
-  ar[0] = r.a[0];  // hi k0..7
-  ar[2] = r.a[1];  // lo k0..7
-  ar[1] = r.a[2];  // hi k8..15
-  ar[3] = r.a[3];  // lo k8..15
-  const int n = tid >> 1, half = tid & 1;
-  uint4*    br = reinterpret_cast<uint4*>(sm.b + n * ROWB);
-  br[half]     = r.b[0];
-  br[2 + half] = r.b[1];
-  if(first) {
-    row_scale(sm.a + tid * ROWB, parity) = r.sa;
-    if(!half) row_scale(sm.b + n * ROWB, parity) = r.sb;
-  }
-}
-
-// MFMA operand fragments of one staged chunk for this wave: rows 64w..64w+63
-// (2 row tiles of 32) and all 4 column tiles, hi and lo halves.
 struct Frags {
-  h8 ahi[2], alo[2], bhi[4], blo[4];
+  h8 bhi[4], blo[4];
 };
 
-__device__ __forceinline__ void read_frags(const MSmem& sm, Frags& F) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int r = lane & 31, kh = (lane >> 5) * 16;
-#pragma unroll
-  for(int i = 0; i < 2; i++) {
-    const unsigned char* p = sm.a + (64 * wave + 32 * i + r) * ROWB + kh;
-    F.ahi[i]               = *reinterpret_cast<const h8*>(p);
-    F.alo[i]               = *reinterpret_cast<const h8*>(p + 32);
-  }
+__device__ __forceinline__ void read_frags(const MSmem& sm, int buf, Frags& F) {
+  const int lane = threadIdx.x & 63, kh = lane >> 5;
 #pragma unroll
   for(int j = 0; j < 4; j++) {
-    const unsigned char* p = sm.b + (32 * j + r) * ROWB + kh;
-    F.bhi[j]               = *reinterpret_cast<const h8*>(p);
-    F.blo[j]               = *reinterpret_cast<const h8*>(p + 32);
+    F.bhi[j] = as_h8(sm.u.g.hi[buf][kh][32 * j + (lane & 31)]);
+    F.blo[j] = as_h8(sm.u.g.lo[buf][kh][32 * j + (lane & 31)]);
   }
 }
 
-// 256x128x16 step on registers: three passes over the 8 accumulators so
+// 64x128x16 of this wave: three passes over its 8 accumulators so that
 // consecutive MFMAs are independent.
-__device__ __forceinline__ void mfma_frags(const Frags& F, f16v (&acc)[2][4]) {
+__device__ __forceinline__ void mfma_chunk(const Slot& s, const Frags& F, f16v (&acc)[2][4]) {
+  const h8 ahi[2] = {as_h8(s.a[0][0]), as_h8(s.a[1][0])};
+  const h8 alo[2] = {as_h8(s.a[0][1]), as_h8(s.a[1][1])};
 #pragma unroll
   for(int j = 0; j < 4; j++)
 #pragma unroll
     for(int i = 0; i < 2; i++)
-      acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(F.ahi[i], F.bhi[j], acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi[i], F.bhi[j], acc[i][j], 0, 0, 0);
 #pragma unroll
   for(int j = 0; j < 4; j++)
 #pragma unroll
     for(int i = 0; i < 2; i++)
-      acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(F.ahi[i], F.blo[j], acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi[i], F.blo[j], acc[i][j], 0, 0, 0);
 #pragma unroll
   for(int j = 0; j < 4; j++)
 #pragma unroll
     for(int i = 0; i < 2; i++)
-      acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(F.alo[i], F.bhi[j], acc[i][j], 0, 0, 0);
+      acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo[i], F.bhi[j], acc[i][j], 0, 0, 0);
 }
 
 // Adds the scaled accumulators of bipartition g into the box and clears them.
-// D layout of v_mfma_f32_32x32x16_f16: register r of lane l holds row
-// 8*(r/4) + 4*(l/32) + r%4, column l%32.
-__device__ __forceinline__ void flush(MSmem& sm, int g, int parity, f16v (&acc)[2][4]) {
+// D layout of v_mfma_f32_32x32x16_f16: register q of lane l holds row
+// 8*(q/4) + 4*(l/32) + q%4, column l%32.  Row scales come from the lane that
+// loaded the row (lane = row within the 32-row tile).
+__device__ __forceinline__ void flush(MSmem& sm, int g, int sslot, const Slot& s,
+                                      f16v (&acc)[2][4]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int hs = g / 3, ps = g % 3;
   int       hA, hB, pA, pB;
   pair_of(hs, hA, hB);
   pair_of(ps, pA, pB);
+  float sb[4];
+  int   en[4];
+#pragma unroll
+  for(int j = 0; j < 4; j++) {
+    const int n = 32 * j + (lane & 31);
+    sb[j]       = sm.u.g.sb[sslot][n];
+    en[j]       = ((n & 3) << sh(hA)) | (((n >> 2) & 3) << sh(hB)) | ((n >> 4) << sh(3 + ps));
+  }
 #pragma unroll
   for(int i = 0; i < 2; i++)
 #pragma unroll
-    for(int j = 0; j < 4; j++) {
-      const int   n  = 32 * j + (lane & 31);
-      const float sb = row_scale(sm.b + n * ROWB, parity);
-      const int   en = ((n & 3) << sh(hA)) | (((n >> 2) & 3) << sh(hB)) | ((n >> 4) << sh(3 + ps));
+    for(int q = 0; q < 16; q++) {
+      const int   r  = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+      const float sa = __shfl(s.sa[i], r, 64);
+      const int   m  = 64 * wave + 32 * i + r;
+      const int   em = ((m & 3) << sh(hs)) | (((m >> 2) & 7) << sh(3 + pA)) | ((m >> 5) << sh(3 + pB));
 #pragma unroll
-      for(int q = 0; q < 16; q++) {
-        const int m  = 64 * wave + 32 * i + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-        const int em = ((m & 3) << sh(hs)) | (((m >> 2) & 7) << sh(3 + pA)) | ((m >> 5) << sh(3 + pB));
-        sm.t3[em | en] += acc[i][j][q] * row_scale(sm.a + m * ROWB, parity) * sb;
+      for(int j = 0; j < 4; j++) {
+        sm.t3[em | en[j]] += acc[i][j][q] * (sa * sb[j]);
         acc[i][j][q] = 0.f;
       }
     }
@@ -271,7 +289,6 @@ __device__ void box_contract(MSmem& sm, const MixedDev& M, uint64_t d1, uint64_t
     for(int j = 0; j < 4; j++)
 #pragma unroll
       for(int q = 0; q < 16; q++) acc[i][j][q] = 0.f;
-  // total chunks and first enabled part
   int total = 0, gp0 = -1;
   for(int gp = 0; gp < 18; gp++)
     if(part_on(d1, d2, gp)) {
@@ -279,64 +296,51 @@ __device__ void box_contract(MSmem& sm, const MixedDev& M, uint64_t d1, uint64_t
       if(gp0 < 0) gp0 = gp;
     }
   if(total == 0) return;
-  // fetch cursor
-  int  fgp = gp0, fch = 0;
+  // fetch cursor (runs PF chunks ahead of the MFMAs)
+  int  fgp = gp0, fch = 0, fpart = 0;
   Part fp  = part_setup(sm, M, fgp, part_code(d1, d2, fgp));
-  auto advance_fetch = [&]() {
+  auto fetch_next = [&](Slot& s) {
+    fetch(fp, fch, 2 * fpart + (fch == 0 ? 1 : 0), s);
     if(++fch == fp.nch) {
       fch = 0;
+      ++fpart;
       do {
         ++fgp;
       } while(fgp < 18 && !part_on(d1, d2, fgp));
       if(fgp < 18) fp = part_setup(sm, M, fgp, part_code(d1, d2, fgp));
     }
   };
-  Ring ring[PF];
+  Slot ring[PF];
 #pragma unroll
   for(int s = 0; s < PF; s++)
-    if(s < total) {
-      fetch(fp, fch, ring[s]);
-      advance_fetch();
-    }
-  // compute cursor.  Software pipeline per chunk c (fragments of c already in
-  // registers F): stage c+1 into LDS, barrier, then issue the MFMAs of c in the
-  // same block as the LDS reads of c+1, barrier.  The matrix pipe works through
-  // the reads; only the write and the two barriers separate consecutive chunks.
+    if(s < total) fetch_next(ring[s]);
+  // compute cursor
   int   cgp = gp0, cch = 0, cn = (cgp & 1) ? M.Kv / KC : M.Ko / KC, cpart = 0;
   Frags F[2];
+  // prologue: chunks 0 and 1 staged, fragments of chunk 0 read
+  __syncthreads();  // previous box's energy sweeps are done with the union area
+  stage(sm, ring[0], 0);
+  if(total > 1) stage(sm, ring[1], 1);
   __syncthreads();
-  stage(sm, ring[0], true, 0);
+  read_frags(sm, 0, F[0]);
   __syncthreads();
-  read_frags(sm, F[0]);
-  if(PF < total) {
-    fetch(fp, fch, ring[0]);
-    advance_fetch();
-  }
-  __syncthreads();
+  // iteration it (chunk it: A in ring[it % PF], B fragments in F[it & 1]):
+  //   read fragments of chunk it+1 (buffer (it+1)&1), MFMAs of chunk it,
+  //   stage chunk it+2 into buffer it&1 (its fragments were read before the
+  //   previous barrier), refill ring[it % PF] with chunk it+PF, flush at the
+  //   end of a part, barrier.
 #pragma unroll 1
-  for(int it0 = 0; it0 < total; it0 += 2 * PF) {
+  for(int it0 = 0; it0 < total; it0 += PF) {
 #pragma unroll
-    for(int s = 0; s < 2 * PF; s++) {
-      const int it = it0 + s;  // chunk in F[s & 1]; chunk it+1 is in ring[(s+1) % PF]
+    for(int s = 0; s < PF; s++) {
+      const int it = it0 + s;
       if(it < total) {
-        const bool more = it + 1 < total;
-        if(more) {
-          const bool nf = cch + 1 == cn;  // chunk it+1 starts the next part
-          stage(sm, ring[(s + 1) % PF], nf, (cpart + (nf ? 1 : 0)) & 1);
-          if(it + 1 + PF < total) {
-            fetch(fp, fch, ring[(s + 1) % PF]);
-            advance_fetch();
-          }
-        }
-        __syncthreads();
-        if(more) read_frags(sm, F[(s + 1) & 1]);
-        mfma_frags(F[s & 1], acc);
-        __syncthreads();
-        if(++cch == cn) {
-          flush(sm, cgp >> 1, cpart & 1, acc);
-          // a one-chunk next part lets a fast wave stage part cpart+2 (same
-          // scale slot) in its next iteration: all flush reads finish first
-          __syncthreads();
+        if(it + 1 < total) read_frags(sm, (s + 1) & 1, F[(s + 1) & 1]);
+        mfma_chunk(ring[s], F[s & 1], acc);
+        if(it + 2 < total) stage(sm, ring[(s + 2) % PF], s & 1);
+        if(++cch == cn) flush(sm, cgp >> 1, cpart % 3, ring[s], acc);
+        if(it + PF < total) fetch_next(ring[s]);
+        if(cch == cn) {
           cch = 0;
           ++cpart;
           do {
@@ -344,119 +348,169 @@ __device__ void box_contract(MSmem& sm, const MixedDev& M, uint64_t d1, uint64_t
           } while(cgp < 18 && !part_on(d1, d2, cgp));
           cn = (cgp & 1) ? M.Kv / KC : M.Ko / KC;
         }
+        __syncthreads();
       }
     }
   }
 }
 
-// Energy of the box.  E4 = sum d^2/D and E5 = E4 + sum d*s/D, with s the s1
-// term; s is linear, so its 9 slots are processed in two passes (slots 0-5,
-// 6-8) whose T1 and <hk hj||pc pb> blocks are staged (FP32) in the free
-// operand staging area.  Slots are compile-time in each pass, so all index
-// arithmetic folds.  Denominators and sums are FP64.
-__device__ __forceinline__ double box_denom(const MSmem& sm, const int* idx) {
-  return sm.eps[8 + idx[2]] + sm.eps[4 + idx[1]] + sm.eps[idx[0]] - sm.eps[28 + idx[5]] -
-         sm.eps[20 + idx[4]] - sm.eps[12 + idx[3]];
+// Denominator reciprocal in FP64: hardware estimate refined by two Newton
+// steps (error below one ulp; |D| is bounded away from zero by the gap).
+__device__ __forceinline__ double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r        = fma(r, e, r);
+  e        = fma(-d, r, 1.0);
+  return fma(r, e, r);
 }
 
-template<int S0, int NS>
-__device__ __forceinline__ double s1_pass(MSmem& sm, const MixedDev& M, uint64_t s1code,
-                                          const int* lim) {
-  static_assert(NS * (32 + 1024) * 4 <= MM * ROWB + NN * ROWB, "s1 staging too large");
-  const int  tid = threadIdx.x;
-  const long O = M.O, V = M.V;
-  float*     T1s = reinterpret_cast<float*>(sm.a);  // [NS][8 pa][4 hi]
-  float*     V1s = T1s + NS * 32;                   // [NS][1024] hk + 4 hj + 16 pc + 128 pb
-  __syncthreads();                                  // staging area free
-#pragma unroll 1
-  for(int q = 0; q < NS; q++) {
-    const int      slot = S0 + q;
-    const unsigned c    = (unsigned) (s1code >> (4 * slot)) & 15u;
-    const int      pa = slot / 3, hi = slot % 3;
-    int            hj, hk, pb, pc;
-    pair_of(hi, hj, hk);
-    pair_of(pa, pb, pc);
-    if(tid < 32) {
-      const int gp = 8 * sm.sub[3 + pa] + (tid >> 2), gh = 4 * sm.sub[hi] + (tid & 3);
-      float     v  = 0.f;
-      if((c & 1) && gp < sm.dim[3 + pa] && gh < sm.dim[hi]) {
-        const double t = M.t1[(long) (sm.off[3 + pa] + gp) * O + sm.off[hi] + gh];
-        v              = (float) ((c & 2) ? -t : t);
-      }
-      T1s[q * 32 + tid] = v;
-    }
-    double x1[4], x2[4];
-#pragma unroll
-    for(int j = 0; j < 4; j++) {
-      const int w = tid + NT * j;
-      const int gk = 4 * sm.sub[hk] + (w & 3), gj = 4 * sm.sub[hj] + ((w >> 2) & 3),
-                gc = 8 * sm.sub[3 + pc] + ((w >> 4) & 7), gb = 8 * sm.sub[3 + pb] + (w >> 7);
-      x1[j] = 0.0;
-      x2[j] = 0.0;
-      if((c & 1) && gk < sm.dim[hk] && gj < sm.dim[hj] && gc < sm.dim[3 + pc] &&
-         gb < sm.dim[3 + pb]) {
-        const long K = sm.off[hk] + gk, J = sm.off[hj] + gj, PC = sm.off[3 + pc] + gc,
-                   PB = sm.off[3 + pb] + gb;
-        if(c & 4) x1[j] = M.vvoo[((PC * V + PB) * O + K) * O + J];
-        if(c & 8) x2[j] = -M.vvoo[((PB * V + PC) * O + K) * O + J];
-      }
-    }
-#pragma unroll
-    for(int j = 0; j < 4; j++) V1s[q * 1024 + tid + NT * j] = (float) (x1[j] + x2[j]);
-  }
-  __syncthreads();
-  double es = 0.0;
-#pragma unroll 1
+// s1 term of S1P slots starting at S0, staged in LDS: returns this thread's
+// sum over its elements of (t3/D) * s.  W = t3/D is already in sm.t3 (zero
+// outside the tile), T1 and V blocks are zero outside the tile.
+template<int S0>
+__device__ __forceinline__ double s1_sweep(const MSmem& sm) {
+  const int tid = threadIdx.x;
+  double    es  = 0.0;
+#pragma unroll 2
   for(int rr = 0; rr < BOX / NT; rr++) {
     const int e = tid + NT * rr;
     int       idx[6];
-    bool      ok = true;
 #pragma unroll
-    for(int p = 0; p < 6; p++) {
-      idx[p] = (e >> sh(p)) & (ext(p) - 1);
-      ok     = ok && idx[p] < lim[p];
-    }
-    if(!ok) continue;
+    for(int p = 0; p < 6; p++) idx[p] = (e >> sh(p)) & (ext(p) - 1);
     float sv = 0.f;
 #pragma unroll
-    for(int q = 0; q < NS; q++) {
+    for(int q = 0; q < S1P; q++) {
       const int slot = S0 + q;
-      const int pa = slot / 3, hi = slot % 3;
-      int       hj, hk, pb, pc;
-      pair_of(hi, hj, hk);
-      pair_of(pa, pb, pc);
-      sv += T1s[q * 32 + idx[3 + pa] * 4 + idx[hi]] *
-            V1s[q * 1024 + idx[hk] + 4 * idx[hj] + 16 * idx[3 + pc] + 128 * idx[3 + pb]];
+      if(slot < 9) {
+        const int pa = slot / 3, hi = slot % 3;
+        int       hj, hk, pb, pc;
+        pair_of(hi, hj, hk);
+        pair_of(pa, pb, pc);
+        sv += sm.u.s.t1[q][idx[3 + pa] * 4 + idx[hi]] *
+              sm.u.s.v1[q][idx[hk] + 4 * idx[hj] + 16 * idx[3 + pc] + 128 * idx[3 + pb]];
+      }
     }
-    es += (double) sm.t3[e] * (double) sv / box_denom(sm, idx);
+    es += (double) (sm.t3[e] * sv);
   }
   return es;
 }
 
+// Loads (into registers) the T1 entries and <hk hj||pc pb> blocks of slots
+// S0..S0+S1P-1.  Consecutive threads read consecutive hj, the contiguous index
+// of vvoo[c][b][k][j].
+struct S1Regs {
+  double v[S1P][4];
+  double t[S1P];
+};
+
+template<int S0>
+__device__ __forceinline__ void s1_load(const MSmem& sm, const MixedDev& M, uint64_t s1code,
+                                        S1Regs& R) {
+  const int  tid = threadIdx.x;
+  const long O = M.O, V = M.V;
+#pragma unroll
+  for(int q = 0; q < S1P; q++) {
+    const int slot = S0 + q;
+    R.t[q]         = 0.0;
+#pragma unroll
+    for(int j = 0; j < 4; j++) R.v[q][j] = 0.0;
+    if(slot >= 9) continue;
+    const unsigned c  = (unsigned) (s1code >> (4 * slot)) & 15u;
+    const int      pa = slot / 3, hi = slot % 3;
+    int            hj, hk, pb, pc;
+    pair_of(hi, hj, hk);
+    pair_of(pa, pb, pc);
+    if(!(c & 1)) continue;
+    if(tid < 32) {
+      const int gp = 8 * sm.sub[3 + pa] + (tid >> 2), gh = 4 * sm.sub[hi] + (tid & 3);
+      if(gp < sm.dim[3 + pa] && gh < sm.dim[hi]) {
+        const double t = M.t1[(long) (sm.off[3 + pa] + gp) * O + sm.off[hi] + gh];
+        R.t[q]         = (c & 2) ? -t : t;
+      }
+    }
+#pragma unroll
+    for(int j = 0; j < 4; j++) {
+      const int w  = tid + NT * j;
+      const int gj = 4 * sm.sub[hj] + (w & 3), gk = 4 * sm.sub[hk] + ((w >> 2) & 3),
+                gc = 8 * sm.sub[3 + pc] + ((w >> 4) & 7), gb = 8 * sm.sub[3 + pb] + (w >> 7);
+      if(gk < sm.dim[hk] && gj < sm.dim[hj] && gc < sm.dim[3 + pc] && gb < sm.dim[3 + pb]) {
+        const long K = sm.off[hk] + gk, J = sm.off[hj] + gj, PC = sm.off[3 + pc] + gc,
+                   PB = sm.off[3 + pb] + gb;
+        double x = 0.0;
+        if(c & 4) x = M.vvoo[((PC * V + PB) * O + K) * O + J];
+        if(c & 8) x -= M.vvoo[((PB * V + PC) * O + K) * O + J];
+        R.v[q][j] = x;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void s1_store(MSmem& sm, const S1Regs& R) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for(int q = 0; q < S1P; q++) {
+    if(tid < 32) sm.u.s.t1[q][tid] = (float) R.t[q];
+#pragma unroll
+    for(int j = 0; j < 4; j++) {
+      const int w = tid + NT * j;  // (j, k, c, b) -> stored as k + 4 j + 16 c + 128 b
+      sm.u.s.v1[q][((w >> 2) & 3) + 4 * (w & 3) + (w & ~15)] = (float) R.v[q][j];
+    }
+  }
+}
+
+// Energy of the box: E4 = sum t3^2/D and E5 = E4 + sum t3*s/D over the tile
+// elements of the box (FP64 sums of per-element terms; t3 and s in FP32).
 __device__ void box_energy(MSmem& sm, const MixedDev& M, uint64_t s1code, double& e4,
                            double& e5) {
   const int tid = threadIdx.x;
-  int       lim[6];
+  S1Regs    R;
+  s1_load<0>(sm, M, s1code, R);  // in flight during the E4 sweep
+  // separable denominators, in the order of the reference's sum
+  for(int x = tid; x < 64 + 512; x += NT) {
+    if(x < 64) sm.u.d.dh[x] = (sm.eps[8 + (x >> 4)] + sm.eps[4 + ((x >> 2) & 3)]) + sm.eps[x & 3];
+    else {
+      const int y    = x - 64;
+      sm.u.d.dp[y] = (sm.eps[28 + (y >> 6)] + sm.eps[20 + ((y >> 3) & 7)]) + sm.eps[12 + (y & 7)];
+    }
+  }
+  __syncthreads();
+  int lim[6];
 #pragma unroll
   for(int p = 0; p < 6; p++) lim[p] = sm.dim[p] - ext(p) * sm.sub[p];
   e4 = 0.0;
 #pragma unroll 2
   for(int rr = 0; rr < BOX / NT; rr++) {
-    const int e = tid + NT * rr;
-    int       idx[6];
+    const int e  = tid + NT * rr;
     bool      ok = true;
 #pragma unroll
-    for(int p = 0; p < 6; p++) {
-      idx[p] = (e >> sh(p)) & (ext(p) - 1);
-      ok     = ok && idx[p] < lim[p];
+    for(int p = 0; p < 6; p++) ok = ok && ((e >> sh(p)) & (ext(p) - 1)) < lim[p];
+    float w = 0.f;
+    if(ok) {
+      const double d   = (double) sm.t3[e];
+      const double inv = recip(sm.u.d.dh[e & 63] - sm.u.d.dp[e >> 6]);
+      const double di  = d * inv;
+      e4 += d * di;
+      w = (float) di;
     }
-    if(!ok) continue;
-    const double d = (double) sm.t3[e];
-    e4 += d * d / box_denom(sm, idx);
+    sm.t3[e] = w;
   }
   double es = 0.0;
-  if(s1code & 0x111111ull) es += s1_pass<0, 6>(sm, M, s1code, lim);
-  if(s1code & 0x111000000ull) es += s1_pass<6, 3>(sm, M, s1code, lim);
+  if(s1code & 0x111111111ull) {
+    __syncthreads();  // denominators no longer needed; W complete
+    s1_store(sm, R);
+    s1_load<S1P>(sm, M, s1code, R);
+    __syncthreads();
+    es += s1_sweep<0>(sm);
+    __syncthreads();
+    s1_store(sm, R);
+    s1_load<2 * S1P>(sm, M, s1code, R);
+    __syncthreads();
+    es += s1_sweep<S1P>(sm);
+    __syncthreads();
+    s1_store(sm, R);
+    __syncthreads();
+    es += s1_sweep<2 * S1P>(sm);
+  }
   e5                  = e4 + es;
   const int lane = tid & 63, wave = tid >> 6;
   e4 = wave_sum(e4);
@@ -486,6 +540,8 @@ __global__ void __launch_bounds__(NT) k_spatial_mixed(MixedDev M, RunDev R) {
       __syncthreads();
       if(tid == 0) {
         const unsigned long long t = atomicAdd(&R.qctr[q], 1ull);
+        // progress mirror in host memory (tickets taken so far, approximate)
+        if(R.prog) __hip_atomic_store(&R.prog[q], t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         if((long long) t >= R.qtotal[q]) s_task = -1;
         else {
           int lo = R.qbeg[q], hi = R.qbeg[q + 1] - 1;
@@ -521,7 +577,10 @@ __global__ void __launch_bounds__(NT) k_spatial_mixed(MixedDev M, RunDev R) {
           sm.dim[p] = T.dim[p];
           sm.off[p] = T.off[p];
         }
-      for(int e = tid; e < BOX; e += NT) sm.t3[e] = 0.f;
+      {
+        float4* t4 = reinterpret_cast<float4*>(sm.t3);
+        for(int e = tid; e < BOX / 4; e += NT) t4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
       __syncthreads();
       if(tid < 36) {
         const int p = tid < 12 ? tid / 4 : 3 + (tid - 12) / 8;
@@ -583,6 +642,12 @@ int build_split(const SpatialBufs& B, int O, int V, SplitBufs& S, void* stream) 
   const double* src[kOperands] = {B.t2h, B.t2ha, B.ovk, B.ovka, B.xv, B.xva, B.t2p, B.t2pa};
   const long    rows[kOperands] = {v * v * o, v * v * o, o * v * o, o * v * o,
                                    v * v * o, v * v * o, o * o * v, o * o * v};
+  {
+    const size_t zb = 4 * (size_t) (ko > kv ? ko : kv);
+    hipError_t   err = hipMalloc((void**) &S.zero, zb);
+    if(err) return (int) err;
+    if((err = hipMemsetAsync(S.zero, 0, zb, st))) return (int) err;
+  }
   for(int j = 0; j < kOperands; j++) {
     const long kin = j < 4 ? op : vp, kout = j < 4 ? ko : kv;
     S.rows[j]      = rows[j];
