@@ -97,11 +97,17 @@ __device__ __forceinline__ h8 as_h8(uint4 u) { return __builtin_bit_cast(h8, u);
 
 // One K part of a bipartition as seen by one thread: the two A rows of its
 // lane (M rows 64w + 32i + lane%32, k half lane/32) and its B column half
-// (column tid/2, k half tid%2), with their scales.
+// (column tid/2, k half tid%2), the addresses of their row exponents and the
+// part's coefficients (sign of the equation and of the chosen term; the same
+// for every thread).  Rows outside the tile read the zero row, so every
+// thread issues the same loads for every chunk: the compiler's wait counts
+// then follow the ring exactly.
 struct Part {
-  const unsigned char* pa[2];  // null: zero row
+  const unsigned char* pa[2];
   const unsigned char* pb;
-  float                sa[2], sb;
+  const short*         ea[2];
+  const short*         eb;
+  float                ca, cb;
   int                  nch;
 };
 
@@ -109,18 +115,30 @@ struct Part {
 struct Slot {
   uint4 a[2][2];  // [row tile i][hi, lo]: 8 k of this lane's k half
   uint4 b[2];     // hi, lo of the staged column half
-  float sa[2], sb;
+  int   ea[2], eb;  // row exponents (scales are 2^-e, applied at flush / stage)
+  float ca, cb;
   int   tag;      // 2 * part ordinal + (first chunk of its part)
 };
 
-// Part gp = 2*g + (d2 ? 1 : 0) of bipartition g for this thread.
+// Part gp = 2*g + (d2 ? 1 : 0) of bipartition g for this thread; gp = 18 is
+// the zero part that pads the chunk count to a multiple of the ring depth.
 __device__ Part part_setup(const MSmem& sm, const MixedDev& M, int gp, unsigned code) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n = tid >> 1, half = tid & 1;
+  Part      P;
+  if(gp >= 18) {
+    P.pa[0] = P.pa[1] = M.zero + 32 * (lane >> 5);
+    P.pb              = M.zero + 32 * half;
+    P.ea[0] = P.ea[1] = P.eb = M.e[0];
+    P.ca = P.cb = 0.f;
+    P.nch       = 1;
+    return P;
+  }
   const int g = gp >> 1, d2 = gp & 1;
   const int hs = g / 3, ps = g % 3;
   int       hA, hB, pA, pB;
   pair_of(hs, hA, hB);
   pair_of(ps, pA, pB);
-  const int  tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int  O = M.O, V = M.V;      // row indices stay below 2^31 (V*V*O rows at most)
   const long K = d2 ? M.Kv : M.Ko;  // padded row length (elements)
   const float sg = (code & 2) ? -1.f : 1.f;
@@ -132,7 +150,6 @@ __device__ Part part_setup(const MSmem& sm, const MixedDev& M, int gp, unsigned 
     row  = hx ? rx : ry;
     coef = (hx || hy) ? (hx ? 1.f : -1.f) : 0.f;
   };
-  Part P;
   P.nch = (int) (K / KC);
 #pragma unroll
   for(int i = 0; i < 2; i++) {
@@ -146,10 +163,11 @@ __device__ Part part_setup(const MSmem& sm, const MixedDev& M, int gp, unsigned 
     float ca;
     if(!d2) choose(code, 4, 8, 1, 0, (PA * V + PB) * O + HS, (PB * V + PA) * O + HS, aa, ra, ca);
     else choose(code, 4, 8, 5, 4, (PB * V + PA) * O + HS, (PA * V + PB) * O + HS, aa, ra, ca);
-    P.pa[i] = (mok && ca != 0.f) ? M.v[aa] + (long) ra * K * 4 + 32 * (lane >> 5) : nullptr;
-    P.sa[i] = P.pa[i] ? sg * ca * ldexpf(1.f, -(int) M.e[aa][ra]) : 0.f;
+    P.pa[i] = (mok && ca != 0.f) ? M.v[aa] + (long) ra * K * 4 + 32 * (lane >> 5)
+                                 : M.zero + 32 * (lane >> 5);
+    P.ea[i] = M.e[aa] + ra;
+    P.ca    = sg * ca;
   }
-  const int  n = tid >> 1, half = tid & 1;
   const int  gHA = 4 * sm.sub[hA] + (n & 3), gHB = 4 * sm.sub[hB] + ((n >> 2) & 3),
              gPS = 8 * sm.sub[3 + ps] + (n >> 4);
   const bool nok = gHA < sm.dim[hA] && gHB < sm.dim[hB] && gPS < sm.dim[3 + ps];
@@ -159,38 +177,28 @@ __device__ Part part_setup(const MSmem& sm, const MixedDev& M, int gp, unsigned 
   float cb;
   if(!d2) choose(code, 16, 32, 3, 2, (HB * V + PS) * O + HA, (HA * V + PS) * O + HB, ab, rb, cb);
   else choose(code, 16, 32, 7, 6, (HA * O + HB) * V + PS, (HB * O + HA) * V + PS, ab, rb, cb);
-  P.pb = (nok && cb != 0.f) ? M.v[ab] + (long) rb * K * 4 + 32 * half : nullptr;
-  P.sb = P.pb ? cb * ldexpf(1.f, -(int) M.e[ab][rb]) : 0.f;
+  P.pb = (nok && cb != 0.f) ? M.v[ab] + (long) rb * K * 4 + 32 * half : M.zero + 32 * half;
+  P.eb = M.e[ab] + rb;
+  P.cb = cb;
   return P;
 }
 
+// Issues the loads of chunk ch of part P (always the same nine loads).
 __device__ __forceinline__ void fetch(const Part& P, int ch, int tag, Slot& s) {
-  const uint4 z = make_uint4(0, 0, 0, 0);
 #pragma unroll
   for(int i = 0; i < 2; i++) {
-    if(P.pa[i]) {
-      const uint4* q = reinterpret_cast<const uint4*>(P.pa[i] + (long) ch * 64);
-      s.a[i][0]      = q[0];
-      s.a[i][1]      = q[1];
-    }
-    else {
-      s.a[i][0] = z;
-      s.a[i][1] = z;
-    }
+    const uint4* q = reinterpret_cast<const uint4*>(P.pa[i] + (long) ch * 64);
+    s.a[i][0]      = q[0];
+    s.a[i][1]      = q[1];
+    s.ea[i]        = *P.ea[i];
   }
-  if(P.pb) {
-    const uint4* q = reinterpret_cast<const uint4*>(P.pb + (long) ch * 64);
-    s.b[0]         = q[0];
-    s.b[1]         = q[1];
-  }
-  else {
-    s.b[0] = z;
-    s.b[1] = z;
-  }
-  s.sa[0] = P.sa[0];
-  s.sa[1] = P.sa[1];
-  s.sb    = P.sb;
-  s.tag   = tag;
+  const uint4* q = reinterpret_cast<const uint4*>(P.pb + (long) ch * 64);
+  s.b[0]         = q[0];
+  s.b[1]         = q[1];
+  s.eb           = *P.eb;
+  s.ca           = P.ca;
+  s.cb           = P.cb;
+  s.tag          = tag;
 }
 
 // Writes a chunk's B column halves into LDS buffer `buf`; the first chunk of
@@ -199,7 +207,7 @@ __device__ __forceinline__ void stage(MSmem& sm, const Slot& s, int buf) {
   const int tid = threadIdx.x, n = tid >> 1, half = tid & 1;
   sm.u.g.hi[buf][half][n] = s.b[0];
   sm.u.g.lo[buf][half][n] = s.b[1];
-  if((s.tag & 1) && !half) sm.u.g.sb[(s.tag >> 1) % 3][n] = s.sb;
+  if((s.tag & 1) && !half) sm.u.g.sb[(s.tag >> 1) % 3][n] = s.cb * ldexpf(1.f, -s.eb);
 }
 
 struct Frags {
@@ -257,11 +265,12 @@ __device__ __forceinline__ void flush(MSmem& sm, int g, int sslot, const Slot& s
     en[j]       = ((n & 3) << sh(hA)) | (((n >> 2) & 3) << sh(hB)) | ((n >> 4) << sh(3 + ps));
   }
 #pragma unroll
-  for(int i = 0; i < 2; i++)
+  for(int i = 0; i < 2; i++) {
+    const float mine = s.ca * ldexpf(1.f, -s.ea[i]);
 #pragma unroll
     for(int q = 0; q < 16; q++) {
       const int   r  = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-      const float sa = __shfl(s.sa[i], r, 64);
+      const float sa = __shfl(mine, r, 64);
       const int   m  = 64 * wave + 32 * i + r;
       const int   em = ((m & 3) << sh(hs)) | (((m >> 2) & 7) << sh(3 + pA)) | ((m >> 5) << sh(3 + pB));
 #pragma unroll
@@ -270,6 +279,7 @@ __device__ __forceinline__ void flush(MSmem& sm, int g, int sslot, const Slot& s
         acc[i][j][q] = 0.f;
       }
     }
+  }
 }
 
 __device__ __forceinline__ bool part_on(uint64_t d1, uint64_t d2, int gp) {
@@ -296,60 +306,62 @@ __device__ void box_contract(MSmem& sm, const MixedDev& M, uint64_t d1, uint64_t
       if(gp0 < 0) gp0 = gp;
     }
   if(total == 0) return;
-  // fetch cursor (runs PF chunks ahead of the MFMAs)
+  // chunk count padded with zero chunks to a multiple of the ring depth, so
+  // the loop body is the same for every chunk
+  const int total_pad = (total + PF - 1) / PF * PF;
+  // fetch cursor (runs PF chunks ahead of the MFMAs); past the last part it
+  // stays on the zero part
   int  fgp = gp0, fch = 0, fpart = 0;
   Part fp  = part_setup(sm, M, fgp, part_code(d1, d2, fgp));
   auto fetch_next = [&](Slot& s) {
-    fetch(fp, fch, 2 * fpart + (fch == 0 ? 1 : 0), s);
+    fetch(fp, fch, fgp < 18 ? 2 * fpart + (fch == 0 ? 1 : 0) : 0, s);
     if(++fch == fp.nch) {
       fch = 0;
-      ++fpart;
-      do {
-        ++fgp;
-      } while(fgp < 18 && !part_on(d1, d2, fgp));
-      if(fgp < 18) fp = part_setup(sm, M, fgp, part_code(d1, d2, fgp));
+      if(fgp < 18) {
+        ++fpart;
+        do {
+          ++fgp;
+        } while(fgp < 18 && !part_on(d1, d2, fgp));
+        fp = part_setup(sm, M, fgp, fgp < 18 ? part_code(d1, d2, fgp) : 0u);
+      }
     }
   };
   Slot ring[PF];
 #pragma unroll
-  for(int s = 0; s < PF; s++)
-    if(s < total) fetch_next(ring[s]);
+  for(int s = 0; s < PF; s++) fetch_next(ring[s]);
   // compute cursor
   int   cgp = gp0, cch = 0, cn = (cgp & 1) ? M.Kv / KC : M.Ko / KC, cpart = 0;
   Frags F[2];
   // prologue: chunks 0 and 1 staged, fragments of chunk 0 read
   __syncthreads();  // previous box's energy sweeps are done with the union area
   stage(sm, ring[0], 0);
-  if(total > 1) stage(sm, ring[1], 1);
+  stage(sm, ring[1], 1);
   __syncthreads();
   read_frags(sm, 0, F[0]);
   __syncthreads();
   // iteration it (chunk it: A in ring[it % PF], B fragments in F[it & 1]):
   //   read fragments of chunk it+1 (buffer (it+1)&1), MFMAs of chunk it,
   //   stage chunk it+2 into buffer it&1 (its fragments were read before the
-  //   previous barrier), refill ring[it % PF] with chunk it+PF, flush at the
-  //   end of a part, barrier.
+  //   previous barrier), flush at the end of a part, refill ring[it % PF]
+  //   with chunk it+PF, barrier.
 #pragma unroll 1
-  for(int it0 = 0; it0 < total; it0 += PF) {
+  for(int it0 = 0; it0 < total_pad; it0 += PF) {
 #pragma unroll
     for(int s = 0; s < PF; s++) {
-      const int it = it0 + s;
-      if(it < total) {
-        if(it + 1 < total) read_frags(sm, (s + 1) & 1, F[(s + 1) & 1]);
-        mfma_chunk(ring[s], F[s & 1], acc);
-        if(it + 2 < total) stage(sm, ring[(s + 2) % PF], s & 1);
-        if(++cch == cn) flush(sm, cgp >> 1, cpart % 3, ring[s], acc);
-        if(it + PF < total) fetch_next(ring[s]);
-        if(cch == cn) {
-          cch = 0;
-          ++cpart;
-          do {
-            ++cgp;
-          } while(cgp < 18 && !part_on(d1, d2, cgp));
-          cn = (cgp & 1) ? M.Kv / KC : M.Ko / KC;
-        }
-        __syncthreads();
+      read_frags(sm, (s + 1) & 1, F[(s + 1) & 1]);
+      mfma_chunk(ring[s], F[s & 1], acc);
+      stage(sm, ring[(s + 2) % PF], s & 1);
+      if(++cch == cn) {
+        flush(sm, cgp >> 1, cpart % 3, ring[s], acc);
+        cch = 0;
+        ++cpart;
+        do {
+          ++cgp;
+        } while(cgp < 18 && !part_on(d1, d2, cgp));
+        cn = cgp < 18 ? ((cgp & 1) ? M.Kv / KC : M.Ko / KC) : 0x7fffffff;
       }
+      fetch_next(ring[s]);
+      __syncthreads();
     }
   }
 }
