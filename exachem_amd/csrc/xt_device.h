@@ -12,6 +12,7 @@ namespace xt {
 struct TaskDev {
   int       dim[6];      // tile extents
   int       off[6];      // first spatial orbital of each tile (occ or virt range)
+  int       tile[6];     // spatial tile index (occupied or virtual tile list) per position
   int       nsub[6];     // ceil(dim/4): 4-wide sub-tiles per position
   int       nbox[6];     // ceil(nsub/2): 2^6-sub-tile boxes per position
   int       n_sub;       // product of nsub
@@ -61,25 +62,29 @@ struct SpatialBufs {
 
 // Split FP16 operands of the mixed-precision kernel.  Array j (same order as
 // the FP64 operand arrays: 0 t2h, 1 t2ha, 2 ovk, 3 ovka, 4 xv, 5 xva, 6 t2p,
-// 7 t2pa) holds, for each contraction run (row) r, a power-of-two exponent
-// e[r] and the scaled values y = x * 2^e[r] split as y ~ hi + lo (FP16 each),
-// stored per 8 consecutive k as [hi x8][lo x8]; rows are padded to Kp16 =
-// roundup(K, 16) with zeros.  max |y| of a row lies in [2^13, 2^14).
+// 7 t2pa) holds, for each contraction run (row) r, the values
+// y = x * 2^e[block(r)] split as y ~ hi + lo (FP16 each), stored per 8
+// consecutive k as [hi x8][lo x8]; rows are padded to Kp16 = roundup(K, 16)
+// with zeros.  A block is the set of rows whose three row coordinates fall in
+// one (tile, tile, tile) triple of the tiling; its exponent puts the block's
+// largest |y| in [2^13, 2^14) (per-tile amplitude scaling), so the scale of a
+// whole contraction part of a task is one number.
 constexpr int kOperands = 8;
 struct SplitBufs {
-  unsigned char* v[kOperands];   // [rows][Kp16/8][2][8] halves
-  short*         e[kOperands];   // [rows]
+  unsigned char* v[kOperands];     // [rows][Kp16/8][2][8] halves
+  int*           eblk[kOperands];  // [blocks] exponents, row-coordinate tile order
   long           rows[kOperands];
-  unsigned char* zero;           // zero row, 4 * max(Kp16) bytes
+  unsigned char* zero;             // zero row, 4 * max(Kp16) bytes
 };
 struct MixedDev {
   int                  O, V, Ko, Kv;  // Ko = roundup(O,16), Kv = roundup(V,16)
+  int                  nto, ntv;      // number of occupied / virtual spatial tiles
   const double*        eo;
   const double*        ev;
   const double*        t1;
   const double*        vvoo;
   const unsigned char* v[kOperands];
-  const short*         e[kOperands];
+  const int*           eblk[kOperands];
   const unsigned char* zero;  // zero row (4 * max(Ko, Kv) bytes): rows outside a tile read it
 };
 
@@ -115,7 +120,9 @@ int launch_spatial_fp64(const SpatialDev& S, const RunDev& R, int grid, void* st
 int launch_reduce(const TaskDev* tasks, int n_tasks, const unsigned long long* acc,
                   double* task_e, double* e45, void* stream);
 int launch_zero_u64(unsigned long long* p, long n, void* stream);
-int build_split(const SpatialBufs& B, int O, int V, SplitBufs& S, void* stream);
+// tile_o[O] / tile_v[V]: spatial tile index of every orbital (host arrays).
+int build_split(const SpatialBufs& B, int O, int V, const int* tile_o, int nto, const int* tile_v,
+                int ntv, SplitBufs& S, void* stream);
 int launch_spatial_mixed(const MixedDev& M, const RunDev& R, int grid, void* stream);
 
 // Task cost model (xt_mlp.hip): 7 inputs (6 tile extents / 48, precision

@@ -267,6 +267,7 @@ int build_plan(Plan& P, int O, int V, const std::vector<int>& occ, const std::ve
       const long b = P.tasks[6 * k + p];
       T.dim[p]     = P.k_range[b];
       T.off[p]     = P.spatial_off[b];
+      T.tile[p]    = p < 3 ? (int) (b % (noab / 2)) : (int) ((b - noab) % (nvab / 2));
       T.nsub[p]    = (T.dim[p] + 3) / 4;
       T.n_sub *= T.nsub[p];
       sp[p] = P.k_spin[b];
@@ -367,9 +368,9 @@ struct xt_ctx {
     drop_graph();
     for(int j = 0; j < xt::kOperands; j++) {
       if(split.v[j]) (void) hipFree(split.v[j]);
-      if(split.e[j]) (void) hipFree(split.e[j]);
+      if(split.eblk[j]) (void) hipFree(split.eblk[j]);
       split.v[j] = nullptr;
-      split.e[j] = nullptr;
+      split.eblk[j] = nullptr;
     }
     if(split.zero) (void) hipFree(split.zero);
     split.zero = nullptr;
@@ -469,6 +470,7 @@ int xt_set_tiling(xt_ctx* c, int O, int V, int n_occ_tiles, const int* occ_tiles
   }
   if(so != O || sv != V) return fail(XT_EINVAL, "xt_set_tiling: tiles do not sum to O / V");
   if(c->have_inputs && (c->plan.O != O || c->plan.V != V)) c->free_inputs();
+  c->free_split();  // block exponents follow the tiling
   int rc = build_plan(c->plan, O, V, occ, vir, is_restricted != 0);
   if(rc) return rc;
   c->have_plan = true;
@@ -770,9 +772,11 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
     M.vvoo = c->in.vvoo;
     for(int j = 0; j < xt::kOperands; j++) {
       M.v[j] = c->split.v[j];
-      M.e[j] = c->split.e[j];
+      M.eblk[j] = c->split.eblk[j];
     }
     M.zero = c->split.zero;
+    M.nto  = c->plan.noab / 2;
+    M.ntv  = c->plan.nvab / 2;
     grid = o.workgroups > 0 ? o.workgroups : c->n_cu;
     grid = (int) std::max<long long>(1, std::min<long long>(grid, tickets));
   }
@@ -844,7 +848,15 @@ static int check_run_state(xt_ctx* c, const xt_opts& o) {
     return fail(XT_EINVAL, "xt_run: unknown precision");
   if(o.precision == XT_PREC_MIXED && !c->have_split) {
     XT_HIP(hipSetDevice(c->device));
-    int err = xt::build_split(c->in, c->plan.O, c->plan.V, c->split, c->stream);
+    // spatial tile of every orbital (the beta half of the tiling mirrors alpha)
+    const Plan&      P = c->plan;
+    std::vector<int> to(P.O), tv(P.V);
+    for(int t = 0; t < P.noab / 2; t++)
+      for(int i = 0; i < P.k_range[t]; i++) to[P.spatial_off[t] + i] = t;
+    for(int t = 0; t < P.nvab / 2; t++)
+      for(int i = 0; i < P.k_range[P.noab + t]; i++) tv[P.spatial_off[P.noab + t] + i] = t;
+    int err = xt::build_split(c->in, P.O, P.V, to.data(), P.noab / 2, tv.data(), P.nvab / 2,
+                              c->split, c->stream);
     if(err) {
       c->free_split();
       return fail(err == (int) hipErrorOutOfMemory ? XT_ENOMEM : XT_EHIP,

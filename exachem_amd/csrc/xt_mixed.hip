@@ -7,18 +7,26 @@
 //   * each bipartition g (single hole hs + particle pair | hole pair +
 //     single particle ps) is a 256 x 128 GEMM: M = (hs,pA,pB) 4*8*8,
 //     N = (hA,hB,ps) 4*4*8, K = h7 (d1) then p7 (d2);
-//   * operands are pre-split FP16 pairs y = hi + lo of x * 2^e (per-row
-//     power-of-two amplitude scale, max |y| in [2^13, 2^14)); every product is
-//     hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_f16 with FP32 accumulation
-//     (~22-bit products); the row scales and the spin-orbital sign are undone
-//     when a K part ends and the result is added into the FP32 box;
-//   * wave w owns M rows 64w..64w+63: their A fragments are loaded from HBM/L2
-//     straight into registers in the MFMA operand layout (no LDS round trip);
-//     the B panel (shared by the four waves) goes through a double-buffered
-//     LDS stage.  A register ring keeps PF chunks of loads in flight across
-//     part and bipartition boundaries, and one barrier per K chunk separates
-//     the B stage of chunk c+2 and the fragment reads of chunk c+1 from the
-//     MFMAs of chunk c;
+//   * operands are pre-split FP16 pairs y = hi + lo of x * 2^e, with one
+//     power-of-two exponent per (tile, tile, tile) block of rows (per-tile
+//     amplitude scaling: the block's max |y| lies in [2^13, 2^14)); every
+//     product is hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_f16 with FP32
+//     accumulation (~22-bit products).  All rows of one contraction part of a
+//     task lie in one block per operand, so the part's scale (both exponents
+//     and the spin-orbital signs) is one number, applied when the part ends and
+//     its result is added into the FP32 box;
+//   * eight waves (two per SIMD, so one wave's LDS and VALU work overlaps the
+//     other's MFMAs); wave w owns the 32 M rows of particle pB = w, loaded
+//     from L2/HBM straight into registers in the MFMA operand layout.  The B
+//     panel (shared by all waves) goes through a double-buffered LDS stage.
+//     A register ring keeps PF chunks of loads in flight across part and
+//     bipartition boundaries; every thread issues the same loads for every
+//     chunk (rows outside the tile read a zero row), so the compiler's wait
+//     counts follow the ring; one barrier per K chunk;
+//   * MFMA tiles wholly outside the task tile (row tile of an out-of-range
+//     pB, column tile of out-of-range ps) are skipped;
+//   * the flush into the box adds precomputed per-lane, per-row and
+//     per-column index parts (disjoint bit fields of the box index);
 //   * energy: one FP64 reciprocal of the denominator per element (from
 //     separable hole and particle sums), E4 in FP64, then t3/D is kept in
 //     place for the s1 sweeps (FP32 products, FP64 sums); task partials are
@@ -33,13 +41,13 @@ namespace {
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float    f16v __attribute__((ext_vector_type(16)));
 
-constexpr int NT  = 256;
-constexpr int MM  = 256;  // bipartition rows   (hs, pA, pB)
-constexpr int NN  = 128;  // bipartition columns (hA, hB, ps)
-constexpr int KC  = 16;   // K per chunk (one MFMA K step)
+constexpr int NT  = 512;
+constexpr int NW  = NT / 64;  // waves; wave w owns M rows 32w..32w+31
+constexpr int NN  = 128;      // bipartition columns (hA, hB, ps)
+constexpr int KC  = 16;       // K per chunk (one MFMA K step)
 constexpr int BOX = 32768;
-constexpr int PF  = 4;    // ring depth: chunks whose loads are in flight per thread
-constexpr int S1P = 3;    // s1 slots staged per sweep
+constexpr int PF  = 4;        // ring depth: chunks whose loads are in flight per thread
+constexpr int S1P = 3;        // s1 slots staged per sweep
 
 struct __align__(16) MSmem {
   float t3[BOX];
@@ -47,7 +55,6 @@ struct __align__(16) MSmem {
     struct {
       uint4 hi[2][2][NN];  // [buffer][k half][column]: FP16 hi of 8 k
       uint4 lo[2][2][NN];  // FP16 lo of the same 8 k
-      float sb[3][NN];     // column scales of parts (ordinal mod 3)
     } g;
     struct {
       double dh[64];   // e_h1 + e_h2 + e_h3 over the box's holes   (index h1 | h2<<2 | h3<<4)
@@ -59,17 +66,19 @@ struct __align__(16) MSmem {
     } s;
   } u;
   double eps[36];  // h1,h2,h3 (4 each) at 0,4,8; p4,p5,p6 (8 each) at 12,20,28
-  double red[8];
-  int    sub[6], dim[6], off[6];
+  double red[2 * NW];
+  int    sub[6], dim[6], off[6], tile[6];
 };
 
 // bit position of each t3 index in the box layout (2 bits per hole, 3 per particle)
-__device__ __forceinline__ int sh(int pos) { return pos < 3 ? 2 * pos : 6 + 3 * (pos - 3); }
-__device__ __forceinline__ int ext(int pos) { return pos < 3 ? 4 : 8; }
+__host__ __device__ constexpr int sh(int pos) { return pos < 3 ? 2 * pos : 6 + 3 * (pos - 3); }
+__host__ __device__ constexpr int ext(int pos) { return pos < 3 ? 4 : 8; }
+__host__ __device__ constexpr int pair_a(int s) { return s == 0 ? 1 : 0; }
+__host__ __device__ constexpr int pair_b(int s) { return s == 2 ? 1 : 2; }
 
 __device__ __forceinline__ void pair_of(int s, int& a, int& b) {
-  a = (s == 0) ? 1 : 0;
-  b = (s == 2) ? 1 : 2;
+  a = pair_a(s);
+  b = pair_b(s);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -95,43 +104,38 @@ __device__ __forceinline__ void fx_add(unsigned long long* acc, double p) {
 
 __device__ __forceinline__ h8 as_h8(uint4 u) { return __builtin_bit_cast(h8, u); }
 
-// One K part of a bipartition as seen by one thread: the two A rows of its
-// lane (M rows 64w + 32i + lane%32, k half lane/32) and its B column half
-// (column tid/2, k half tid%2), the addresses of their row exponents and the
-// part's coefficients (sign of the equation and of the chosen term; the same
-// for every thread).  Rows outside the tile read the zero row, so every
-// thread issues the same loads for every chunk: the compiler's wait counts
-// then follow the ring exactly.
+// One K part of a bipartition as seen by one thread: the A row of its lane
+// (M row 32w + lane%32, k half lane/32), its 16-byte piece of a B column
+// (column tid/4, piece tid%4 = [hi k0-7 | lo k0-7 | hi k8-15 | lo k8-15]) and
+// the part's scale (the same for every thread).  Rows outside the tile read
+// the zero row, so every thread issues the same loads for every chunk and the
+// compiler's wait counts follow the ring.
 struct Part {
-  const unsigned char* pa[2];
+  const unsigned char* pa;
   const unsigned char* pb;
-  const short*         ea[2];
-  const short*         eb;
-  float                ca, cb;
+  float                sc;  // sign * 2^-(eA + eB)
   int                  nch;
 };
 
 // One ring slot: the loads of one K chunk plus what its consumers need.
 struct Slot {
-  uint4 a[2][2];  // [row tile i][hi, lo]: 8 k of this lane's k half
-  uint4 b[2];     // hi, lo of the staged column half
-  int   ea[2], eb;  // row exponents (scales are 2^-e, applied at flush / stage)
-  float ca, cb;
-  int   tag;      // 2 * part ordinal + (first chunk of its part)
+  uint4 a[2];  // hi, lo of this lane's 8 k
+  uint4 b;     // this thread's B piece
+  float sc;    // scale of the chunk's part
 };
 
 // Part gp = 2*g + (d2 ? 1 : 0) of bipartition g for this thread; gp = 18 is
 // the zero part that pads the chunk count to a multiple of the ring depth.
-__device__ Part part_setup(const MSmem& sm, const MixedDev& M, int gp, unsigned code) {
+__device__ __forceinline__ Part part_setup(const MSmem& sm, const MixedDev& M, int gp,
+                                           unsigned code) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int n = tid >> 1, half = tid & 1;
+  const int n = tid >> 2, piece = tid & 3;
   Part      P;
   if(gp >= 18) {
-    P.pa[0] = P.pa[1] = M.zero + 32 * (lane >> 5);
-    P.pb              = M.zero + 32 * half;
-    P.ea[0] = P.ea[1] = P.eb = M.e[0];
-    P.ca = P.cb = 0.f;
-    P.nch       = 1;
+    P.pa  = M.zero + 32 * (lane >> 5);
+    P.pb  = M.zero + 16 * piece;
+    P.sc  = 0.f;
+    P.nch = 1;
     return P;
   }
   const int g = gp >> 1, d2 = gp & 1;
@@ -140,74 +144,74 @@ __device__ Part part_setup(const MSmem& sm, const MixedDev& M, int gp, unsigned 
   pair_of(hs, hA, hB);
   pair_of(ps, pA, pB);
   const int  O = M.O, V = M.V;      // row indices stay below 2^31 (V*V*O rows at most)
+  const int  no = M.nto, nv = M.ntv;
   const long K = d2 ? M.Kv : M.Ko;  // padded row length (elements)
-  const float sg = (code & 2) ? -1.f : 1.f;
-  // row index and operand array of c1*x - c2*y (both -> antisymmetrised array)
+  // operand array, row, block and coefficient of c1*x - c2*y (both terms ->
+  // the antisymmetrised array, which holds x - y)
   auto choose = [](unsigned c, unsigned bx, unsigned by, int both, int single, int rx, int ry,
-                   int& arr, int& row, float& coef) {
+                   int kx, int ky, int& arr, int& row, int& blk, float& coef) {
     const bool hx = (c & bx) != 0, hy = (c & by) != 0;
     arr  = hx && hy ? both : single;
     row  = hx ? rx : ry;
+    blk  = hx ? kx : ky;
     coef = (hx || hy) ? (hx ? 1.f : -1.f) : 0.f;
   };
+  const int tHS = sm.tile[hs], tHA = sm.tile[hA], tHB = sm.tile[hB];
+  const int tPS = sm.tile[3 + ps], tPA = sm.tile[3 + pA], tPB = sm.tile[3 + pB];
   P.nch = (int) (K / KC);
-#pragma unroll
-  for(int i = 0; i < 2; i++) {
-    const int  m   = 64 * wave + 32 * i + (lane & 31);
+  int   aa, ra, ka, ab, rb, kb;
+  float ca, cb;
+  {
+    const int  m   = 32 * wave + (lane & 31);
     const int  gHS = 4 * sm.sub[hs] + (m & 3), gPA = 8 * sm.sub[3 + pA] + ((m >> 2) & 7),
                gPB = 8 * sm.sub[3 + pB] + (m >> 5);
     const bool mok = gHS < sm.dim[hs] && gPA < sm.dim[3 + pA] && gPB < sm.dim[3 + pB];
     const int HS = sm.off[hs] + (mok ? gHS : 0), PA = sm.off[3 + pA] + (mok ? gPA : 0),
               PB = sm.off[3 + pB] + (mok ? gPB : 0);
-    int   aa, ra;
-    float ca;
-    if(!d2) choose(code, 4, 8, 1, 0, (PA * V + PB) * O + HS, (PB * V + PA) * O + HS, aa, ra, ca);
-    else choose(code, 4, 8, 5, 4, (PB * V + PA) * O + HS, (PA * V + PB) * O + HS, aa, ra, ca);
-    P.pa[i] = (mok && ca != 0.f) ? M.v[aa] + (long) ra * K * 4 + 32 * (lane >> 5)
-                                 : M.zero + 32 * (lane >> 5);
-    P.ea[i] = M.e[aa] + ra;
-    P.ca    = sg * ca;
+    if(!d2)  // t2h[a][b][i]: rows (pA,pB,hs) or (pB,pA,hs), tiles (V,V,O)
+      choose(code, 4, 8, 1, 0, (PA * V + PB) * O + HS, (PB * V + PA) * O + HS,
+             (tPA * nv + tPB) * no + tHS, (tPB * nv + tPA) * no + tHS, aa, ra, ka, ca);
+    else  // xv[c][b][i]: rows (pB,pA,hs) or (pA,pB,hs), tiles (V,V,O)
+      choose(code, 4, 8, 5, 4, (PB * V + PA) * O + HS, (PA * V + PB) * O + HS,
+             (tPB * nv + tPA) * no + tHS, (tPA * nv + tPB) * no + tHS, aa, ra, ka, ca);
+    P.pa = (mok && ca != 0.f) ? M.v[aa] + (long) ra * K * 4 + 32 * (lane >> 5)
+                              : M.zero + 32 * (lane >> 5);
   }
-  const int  gHA = 4 * sm.sub[hA] + (n & 3), gHB = 4 * sm.sub[hB] + ((n >> 2) & 3),
-             gPS = 8 * sm.sub[3 + ps] + (n >> 4);
-  const bool nok = gHA < sm.dim[hA] && gHB < sm.dim[hB] && gPS < sm.dim[3 + ps];
-  const int HA = sm.off[hA] + (nok ? gHA : 0), HB = sm.off[hB] + (nok ? gHB : 0),
-            PS = sm.off[3 + ps] + (nok ? gPS : 0);
-  int   ab, rb;
-  float cb;
-  if(!d2) choose(code, 16, 32, 3, 2, (HB * V + PS) * O + HA, (HA * V + PS) * O + HB, ab, rb, cb);
-  else choose(code, 16, 32, 7, 6, (HA * O + HB) * V + PS, (HB * O + HA) * V + PS, ab, rb, cb);
-  P.pb = (nok && cb != 0.f) ? M.v[ab] + (long) rb * K * 4 + 32 * half : M.zero + 32 * half;
-  P.eb = M.e[ab] + rb;
-  P.cb = cb;
+  {
+    const int  gHA = 4 * sm.sub[hA] + (n & 3), gHB = 4 * sm.sub[hB] + ((n >> 2) & 3),
+               gPS = 8 * sm.sub[3 + ps] + (n >> 4);
+    const bool nok = gHA < sm.dim[hA] && gHB < sm.dim[hB] && gPS < sm.dim[3 + ps];
+    const int HA = sm.off[hA] + (nok ? gHA : 0), HB = sm.off[hB] + (nok ? gHB : 0),
+              PS = sm.off[3 + ps] + (nok ? gPS : 0);
+    if(!d2)  // ovk[i][a][j]: rows (hB,ps,hA) or (hA,ps,hB), tiles (O,V,O)
+      choose(code, 16, 32, 3, 2, (HB * V + PS) * O + HA, (HA * V + PS) * O + HB,
+             (tHB * nv + tPS) * no + tHA, (tHA * nv + tPS) * no + tHB, ab, rb, kb, cb);
+    else  // t2p[i][j][a]: rows (hA,hB,ps) or (hB,hA,ps), tiles (O,O,V)
+      choose(code, 16, 32, 7, 6, (HA * O + HB) * V + PS, (HB * O + HA) * V + PS,
+             (tHA * no + tHB) * nv + tPS, (tHB * no + tHA) * nv + tPS, ab, rb, kb, cb);
+    P.pb = (nok && cb != 0.f) ? M.v[ab] + (long) rb * K * 4 + 16 * piece : M.zero + 16 * piece;
+  }
+  // block exponents: the same address in every lane (scalar loads)
+  const int   ea = M.eblk[aa][ka], eb = M.eblk[ab][kb];
+  const float sg = (code & 2) ? -1.f : 1.f;
+  P.sc           = sg * ca * cb * ldexpf(1.f, -(ea + eb));
   return P;
 }
 
-// Issues the loads of chunk ch of part P (always the same nine loads).
-__device__ __forceinline__ void fetch(const Part& P, int ch, int tag, Slot& s) {
-#pragma unroll
-  for(int i = 0; i < 2; i++) {
-    const uint4* q = reinterpret_cast<const uint4*>(P.pa[i] + (long) ch * 64);
-    s.a[i][0]      = q[0];
-    s.a[i][1]      = q[1];
-    s.ea[i]        = *P.ea[i];
-  }
-  const uint4* q = reinterpret_cast<const uint4*>(P.pb + (long) ch * 64);
-  s.b[0]         = q[0];
-  s.b[1]         = q[1];
-  s.eb           = *P.eb;
-  s.ca           = P.ca;
-  s.cb           = P.cb;
-  s.tag          = tag;
+// Issues the loads of chunk ch of part P (always the same three loads).
+__device__ __forceinline__ void fetch(const Part& P, int ch, Slot& s) {
+  const uint4* q = reinterpret_cast<const uint4*>(P.pa + (long) ch * 64);
+  s.a[0]         = q[0];
+  s.a[1]         = q[1];
+  s.b            = *reinterpret_cast<const uint4*>(P.pb + (long) ch * 64);
+  s.sc           = P.sc;
 }
 
-// Writes a chunk's B column halves into LDS buffer `buf`; the first chunk of
-// a part also publishes the part's column scales.
+// Writes a chunk's B pieces into LDS buffer `buf`.
 __device__ __forceinline__ void stage(MSmem& sm, const Slot& s, int buf) {
-  const int tid = threadIdx.x, n = tid >> 1, half = tid & 1;
-  sm.u.g.hi[buf][half][n] = s.b[0];
-  sm.u.g.lo[buf][half][n] = s.b[1];
-  if((s.tag & 1) && !half) sm.u.g.sb[(s.tag >> 1) % 3][n] = s.cb * ldexpf(1.f, -s.eb);
+  const int tid = threadIdx.x, n = tid >> 2, piece = tid & 3;
+  uint4*    plane = (piece & 1) ? &sm.u.g.lo[buf][piece >> 1][n] : &sm.u.g.hi[buf][piece >> 1][n];
+  *plane          = s.b;
 }
 
 struct Frags {
@@ -223,61 +227,46 @@ __device__ __forceinline__ void read_frags(const MSmem& sm, int buf, Frags& F) {
   }
 }
 
-// 64x128x16 of this wave: three passes over its 8 accumulators so that
-// consecutive MFMAs are independent.
-__device__ __forceinline__ void mfma_chunk(const Slot& s, const Frags& F, f16v (&acc)[2][4]) {
-  const h8 ahi[2] = {as_h8(s.a[0][0]), as_h8(s.a[1][0])};
-  const h8 alo[2] = {as_h8(s.a[0][1]), as_h8(s.a[1][1])};
+// 32x128x16 of this wave: three passes over its 4 accumulators; column
+// tiles outside the task tile (cmask bit clear) are skipped.
+__device__ __forceinline__ void mfma_chunk(const Slot& s, const Frags& F, unsigned cmask,
+                                           f16v (&acc)[4]) {
+  const h8 ahi = as_h8(s.a[0]), alo = as_h8(s.a[1]);
 #pragma unroll
   for(int j = 0; j < 4; j++)
-#pragma unroll
-    for(int i = 0; i < 2; i++)
-      acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi[i], F.bhi[j], acc[i][j], 0, 0, 0);
+    if(cmask & (1u << j)) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, F.bhi[j], acc[j], 0, 0, 0);
 #pragma unroll
   for(int j = 0; j < 4; j++)
-#pragma unroll
-    for(int i = 0; i < 2; i++)
-      acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi[i], F.blo[j], acc[i][j], 0, 0, 0);
+    if(cmask & (1u << j)) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, F.blo[j], acc[j], 0, 0, 0);
 #pragma unroll
   for(int j = 0; j < 4; j++)
-#pragma unroll
-    for(int i = 0; i < 2; i++)
-      acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo[i], F.bhi[j], acc[i][j], 0, 0, 0);
+    if(cmask & (1u << j)) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, F.bhi[j], acc[j], 0, 0, 0);
 }
 
 // Adds the scaled accumulators of bipartition g into the box and clears them.
 // D layout of v_mfma_f32_32x32x16_f16: register q of lane l holds row
-// 8*(q/4) + 4*(l/32) + q%4, column l%32.  Row scales come from the lane that
-// loaded the row (lane = row within the 32-row tile).
-__device__ __forceinline__ void flush(MSmem& sm, int g, int sslot, const Slot& s,
-                                      f16v (&acc)[2][4]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// 8*(q/4) + 4*(l/32) + q%4, column l%32.  The t3 index of (q, j) is a
+// per-lane base plus a q part plus a j part (disjoint bit fields), each
+// computed once per flush; the part's scale is one number.
+__device__ __forceinline__ void flush(MSmem& sm, int g, float sc, unsigned cmask, f16v (&acc)[4]) {
   const int hs = g / 3, ps = g % 3;
-  int       hA, hB, pA, pB;
-  pair_of(hs, hA, hB);
-  pair_of(ps, pA, pB);
-  float sb[4];
-  int   en[4];
+  const int hA = pair_a(hs), hB = pair_b(hs), pA = pair_a(ps), pB = pair_b(ps);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int shs = sh(hs), spa = sh(3 + pA), sps = sh(3 + ps);
+  // m = 32 wave + 8 (q/4) + 4 (lane/32) + q%4: hs = m & 3, pA = (m >> 2) & 7, pB = m >> 5
+  // n = 32 j + lane%32:                       hA = n & 3, hB = (n >> 2) & 3, ps = n >> 4
+  const int base = ((lane >> 5) << spa) | (wave << sh(3 + pB)) | ((lane & 3) << sh(hA)) |
+                   (((lane >> 2) & 3) << sh(hB)) | (((lane >> 4) & 1) << sps);
+  int joff[4];
 #pragma unroll
-  for(int j = 0; j < 4; j++) {
-    const int n = 32 * j + (lane & 31);
-    sb[j]       = sm.u.g.sb[sslot][n];
-    en[j]       = ((n & 3) << sh(hA)) | (((n >> 2) & 3) << sh(hB)) | ((n >> 4) << sh(3 + ps));
-  }
+  for(int j = 0; j < 4; j++) joff[j] = base + ((2 * j) << sps);
 #pragma unroll
-  for(int i = 0; i < 2; i++) {
-    const float mine = s.ca * ldexpf(1.f, -s.ea[i]);
+  for(int q = 0; q < 16; q++) {
+    const int qoff = ((q & 3) << shs) | ((2 * (q >> 2)) << spa);
 #pragma unroll
-    for(int q = 0; q < 16; q++) {
-      const int   r  = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-      const float sa = __shfl(mine, r, 64);
-      const int   m  = 64 * wave + 32 * i + r;
-      const int   em = ((m & 3) << sh(hs)) | (((m >> 2) & 7) << sh(3 + pA)) | ((m >> 5) << sh(3 + pB));
-#pragma unroll
-      for(int j = 0; j < 4; j++) {
-        sm.t3[em | en[j]] += acc[i][j][q] * (sa * sb[j]);
-        acc[i][j][q] = 0.f;
-      }
+    for(int j = 0; j < 4; j++) {
+      if(cmask & (1u << j)) sm.t3[joff[j] + qoff] += acc[j][q] * sc;
+      acc[j][q] = 0.f;
     }
   }
 }
@@ -291,14 +280,23 @@ __device__ __forceinline__ unsigned part_code(uint64_t d1, uint64_t d2, int gp) 
   return (unsigned) (c >> (6 * (gp >> 1))) & 63u;
 }
 
-__device__ void box_contract(MSmem& sm, const MixedDev& M, uint64_t d1, uint64_t d2) {
-  f16v acc[2][4];
+// Column tiles of bipartition g holding a particle ps index inside the tile
+// (tile j covers ps = 2j, 2j+1), or none when this wave's row tile (pB = wave)
+// lies outside.
+__device__ __forceinline__ unsigned tile_mask(const MSmem& sm, int g) {
+  const int ps = g % 3, pB = pair_b(ps), wave = threadIdx.x >> 6;
+  if(8 * sm.sub[3 + pB] + wave >= sm.dim[3 + pB]) return 0u;
+  const int left = sm.dim[3 + ps] - 8 * sm.sub[3 + ps];  // valid ps in this box (>= 1)
+  const int nj   = left >= 8 ? 4 : (left + 1) >> 1;
+  return (1u << nj) - 1u;
+}
+
+__device__ __forceinline__ void box_contract(MSmem& sm, const MixedDev& M, uint64_t d1, uint64_t d2) {
+  f16v acc[4];
 #pragma unroll
-  for(int i = 0; i < 2; i++)
+  for(int j = 0; j < 4; j++)
 #pragma unroll
-    for(int j = 0; j < 4; j++)
-#pragma unroll
-      for(int q = 0; q < 16; q++) acc[i][j][q] = 0.f;
+    for(int q = 0; q < 16; q++) acc[j][q] = 0.f;
   int total = 0, gp0 = -1;
   for(int gp = 0; gp < 18; gp++)
     if(part_on(d1, d2, gp)) {
@@ -311,14 +309,13 @@ __device__ void box_contract(MSmem& sm, const MixedDev& M, uint64_t d1, uint64_t
   const int total_pad = (total + PF - 1) / PF * PF;
   // fetch cursor (runs PF chunks ahead of the MFMAs); past the last part it
   // stays on the zero part
-  int  fgp = gp0, fch = 0, fpart = 0;
+  int  fgp = gp0, fch = 0;
   Part fp  = part_setup(sm, M, fgp, part_code(d1, d2, fgp));
   auto fetch_next = [&](Slot& s) {
-    fetch(fp, fch, fgp < 18 ? 2 * fpart + (fch == 0 ? 1 : 0) : 0, s);
+    fetch(fp, fch, s);
     if(++fch == fp.nch) {
       fch = 0;
       if(fgp < 18) {
-        ++fpart;
         do {
           ++fgp;
         } while(fgp < 18 && !part_on(d1, d2, fgp));
@@ -330,8 +327,9 @@ __device__ void box_contract(MSmem& sm, const MixedDev& M, uint64_t d1, uint64_t
 #pragma unroll
   for(int s = 0; s < PF; s++) fetch_next(ring[s]);
   // compute cursor
-  int   cgp = gp0, cch = 0, cn = (cgp & 1) ? M.Kv / KC : M.Ko / KC, cpart = 0;
-  Frags F[2];
+  int      cgp = gp0, cch = 0, cn = (cgp & 1) ? M.Kv / KC : M.Ko / KC;
+  unsigned cmask = __builtin_amdgcn_readfirstlane(tile_mask(sm, cgp >> 1));
+  Frags    F[2];
   // prologue: chunks 0 and 1 staged, fragments of chunk 0 read
   __syncthreads();  // previous box's energy sweeps are done with the union area
   stage(sm, ring[0], 0);
@@ -340,25 +338,25 @@ __device__ void box_contract(MSmem& sm, const MixedDev& M, uint64_t d1, uint64_t
   read_frags(sm, 0, F[0]);
   __syncthreads();
   // iteration it (chunk it: A in ring[it % PF], B fragments in F[it & 1]):
-  //   read fragments of chunk it+1 (buffer (it+1)&1), MFMAs of chunk it,
-  //   stage chunk it+2 into buffer it&1 (its fragments were read before the
-  //   previous barrier), flush at the end of a part, refill ring[it % PF]
-  //   with chunk it+PF, barrier.
+  //   read fragments of chunk it+1 (buffer (it+1)&1) while the MFMAs of
+  //   chunk it run, stage chunk it+2 into buffer it&1 (its fragments were
+  //   read before the previous barrier), flush at the end of a part, refill
+  //   ring[it % PF] with chunk it+PF, barrier.
 #pragma unroll 1
   for(int it0 = 0; it0 < total_pad; it0 += PF) {
 #pragma unroll
     for(int s = 0; s < PF; s++) {
       read_frags(sm, (s + 1) & 1, F[(s + 1) & 1]);
-      mfma_chunk(ring[s], F[s & 1], acc);
+      mfma_chunk(ring[s], F[s & 1], cmask, acc);
       stage(sm, ring[(s + 2) % PF], s & 1);
       if(++cch == cn) {
-        flush(sm, cgp >> 1, cpart % 3, ring[s], acc);
+        if(cmask) flush(sm, cgp >> 1, ring[s].sc, cmask, acc);
         cch = 0;
-        ++cpart;
         do {
           ++cgp;
         } while(cgp < 18 && !part_on(d1, d2, cgp));
-        cn = cgp < 18 ? ((cgp & 1) ? M.Kv / KC : M.Ko / KC) : 0x7fffffff;
+        cn    = cgp < 18 ? ((cgp & 1) ? M.Kv / KC : M.Ko / KC) : 0x7fffffff;
+        cmask = __builtin_amdgcn_readfirstlane(cgp < 18 ? tile_mask(sm, cgp >> 1) : 0u);
       }
       fetch_next(ring[s]);
       __syncthreads();
@@ -410,8 +408,9 @@ __device__ __forceinline__ double s1_sweep(const MSmem& sm) {
 // Loads (into registers) the T1 entries and <hk hj||pc pb> blocks of slots
 // S0..S0+S1P-1.  Consecutive threads read consecutive hj, the contiguous index
 // of vvoo[c][b][k][j].
+constexpr int S1V = 1024 / NT;  // block elements per thread and slot
 struct S1Regs {
-  double v[S1P][4];
+  double v[S1P][S1V];
   double t[S1P];
 };
 
@@ -425,7 +424,7 @@ __device__ __forceinline__ void s1_load(const MSmem& sm, const MixedDev& M, uint
     const int slot = S0 + q;
     R.t[q]         = 0.0;
 #pragma unroll
-    for(int j = 0; j < 4; j++) R.v[q][j] = 0.0;
+    for(int j = 0; j < S1V; j++) R.v[q][j] = 0.0;
     if(slot >= 9) continue;
     const unsigned c  = (unsigned) (s1code >> (4 * slot)) & 15u;
     const int      pa = slot / 3, hi = slot % 3;
@@ -441,7 +440,7 @@ __device__ __forceinline__ void s1_load(const MSmem& sm, const MixedDev& M, uint
       }
     }
 #pragma unroll
-    for(int j = 0; j < 4; j++) {
+    for(int j = 0; j < S1V; j++) {
       const int w  = tid + NT * j;
       const int gj = 4 * sm.sub[hj] + (w & 3), gk = 4 * sm.sub[hk] + ((w >> 2) & 3),
                 gc = 8 * sm.sub[3 + pc] + ((w >> 4) & 7), gb = 8 * sm.sub[3 + pb] + (w >> 7);
@@ -463,7 +462,7 @@ __device__ __forceinline__ void s1_store(MSmem& sm, const S1Regs& R) {
   for(int q = 0; q < S1P; q++) {
     if(tid < 32) sm.u.s.t1[q][tid] = (float) R.t[q];
 #pragma unroll
-    for(int j = 0; j < 4; j++) {
+    for(int j = 0; j < S1V; j++) {
       const int w = tid + NT * j;  // (j, k, c, b) -> stored as k + 4 j + 16 c + 128 b
       sm.u.s.v1[q][((w >> 2) & 3) + 4 * (w & 3) + (w & ~15)] = (float) R.v[q][j];
     }
@@ -472,7 +471,7 @@ __device__ __forceinline__ void s1_store(MSmem& sm, const S1Regs& R) {
 
 // Energy of the box: E4 = sum t3^2/D and E5 = E4 + sum t3*s/D over the tile
 // elements of the box (FP64 sums of per-element terms; t3 and s in FP32).
-__device__ void box_energy(MSmem& sm, const MixedDev& M, uint64_t s1code, double& e4,
+__device__ __forceinline__ void box_energy(MSmem& sm, const MixedDev& M, uint64_t s1code, double& e4,
                            double& e5) {
   const int tid = threadIdx.x;
   S1Regs    R;
@@ -528,13 +527,19 @@ __device__ void box_energy(MSmem& sm, const MixedDev& M, uint64_t s1code, double
   e4 = wave_sum(e4);
   e5 = wave_sum(e5);
   if(lane == 0) {
-    sm.red[wave]     = e4;
-    sm.red[4 + wave] = e5;
+    sm.red[wave]      = e4;
+    sm.red[NW + wave] = e5;
   }
   __syncthreads();
   if(tid == 0) {
-    e4 = ((sm.red[0] + sm.red[1]) + sm.red[2]) + sm.red[3];
-    e5 = ((sm.red[4] + sm.red[5]) + sm.red[6]) + sm.red[7];
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for(int w = 0; w < NW; w++) {
+      a += sm.red[w];
+      b += sm.red[NW + w];
+    }
+    e4 = a;
+    e5 = b;
   }
 }
 
@@ -587,7 +592,8 @@ __global__ void __launch_bounds__(NT) k_spatial_mixed(MixedDev M, RunDev R) {
         if(tid == p) {
           sm.sub[p] = sub[p];
           sm.dim[p] = T.dim[p];
-          sm.off[p] = T.off[p];
+          sm.off[p]  = T.off[p];
+          sm.tile[p] = T.tile[p];
         }
       {
         float4* t4 = reinterpret_cast<float4*>(sm.t3);
@@ -614,7 +620,10 @@ __global__ void __launch_bounds__(NT) k_spatial_mixed(MixedDev M, RunDev R) {
   }
 }
 
-// Per-row exponent: one wave per row, e = 13 - floor(log2(max|x|)).
+// Per-row exponent: one wave per row, e = 13 - floor(log2(max|x|)); an
+// all-zero row puts no constraint on its block (kNoRow).
+constexpr int kNoRow = 200;
+
 __global__ void k_row_exp(const double* __restrict__ x, long rows, long Kin, short* __restrict__ e) {
   const long row  = (long) blockIdx.x * 4 + (threadIdx.x >> 6);
   const int  lane = threadIdx.x & 63;
@@ -624,57 +633,118 @@ __global__ void k_row_exp(const double* __restrict__ x, long rows, long Kin, sho
 #pragma unroll
   for(int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
   if(lane == 0) {
-    int ex = 0;
+    int ex = kNoRow;
     if(m > 0.0) ex = 13 - ilogb(m);
-    ex     = ex > 200 ? 200 : (ex < -200 ? -200 : ex);
+    ex     = ex > kNoRow ? kNoRow : (ex < -kNoRow ? -kNoRow : ex);
     e[row] = (short) ex;
   }
 }
 
-// Split: y = x * 2^e[row] = hi + lo (FP16), written per 8 k as [hi8][lo8].
-__global__ void k_split(const double* __restrict__ x, long rows, long Kin, long Kout,
-                        const short* __restrict__ e, _Float16* __restrict__ out) {
+// Row r = (x*D2 + y)*D3 + z of an operand array; each coordinate is an
+// occupied (type 0) or virtual (type 1) orbital.  Its block is the tile
+// triple of (x, y, z) in the same order.
+struct BlockMap {
+  long       D2, D3;
+  int        n2, n3;                  // tiles of coordinates 2 and 3
+  const int* t1;
+  const int* t2;
+  const int* t3;                      // orbital -> tile maps of the three coordinates
+  __device__ long operator()(long r) const {
+    const long z = r % D3, y = (r / D3) % D2, x = r / (D2 * D3);
+    return ((long) t1[x] * n2 + t2[y]) * n3 + t3[z];
+  }
+};
+
+__global__ void k_fill_i32(int* p, long n, int v) {
+  const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) p[i] = v;
+}
+
+// Block exponent = min over the block's rows (the largest row sets it).
+__global__ void k_block_exp(const short* __restrict__ er, long rows, BlockMap bm, int* eb) {
+  const long r = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if(r >= rows) return;
+  const int e = er[r];
+  if(e < kNoRow) atomicMin(&eb[bm(r)], e);
+}
+
+// Split: y = x * 2^e[block(row)] = hi + lo (FP16), written per 8 k as [hi8][lo8].
+__global__ void k_split(const double* __restrict__ x, long rows, long Kin, long Kout, BlockMap bm,
+                        const int* __restrict__ eb, _Float16* __restrict__ out) {
   const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= rows * Kout) return;
-  const long   row = i / Kout, k = i % Kout;
-  const double y   = k < Kin ? ldexp(x[row * Kin + k], (int) e[row]) : 0.0;
-  const _Float16 h = (_Float16) y;
-  const _Float16 l = (_Float16) (y - (double) h);
-  const long   base = row * Kout * 2 + (k / 8) * 16 + (k % 8);
+  const long     row = i / Kout, k = i % Kout;
+  const double   y   = k < Kin ? ldexp(x[row * Kin + k], eb[bm(row)]) : 0.0;
+  const _Float16 h   = (_Float16) y;
+  const _Float16 l   = (_Float16) (y - (double) h);
+  const long     base = row * Kout * 2 + (k / 8) * 16 + (k % 8);
   out[base]     = h;
   out[base + 8] = l;
 }
 
 }  // namespace
 
-int build_split(const SpatialBufs& B, int O, int V, SplitBufs& S, void* stream) {
-  hipStream_t   st  = (hipStream_t) stream;
+int build_split(const SpatialBufs& B, int O, int V, const int* tile_o, int nto, const int* tile_v,
+                int ntv, SplitBufs& S, void* stream) {
+  hipStream_t   st = (hipStream_t) stream;
   const long    o = O, v = V, op = (O + 7) / 8 * 8, vp = (V + 7) / 8 * 8;
   const long    ko = (O + 15) / 16 * 16, kv = (V + 15) / 16 * 16;
   const double* src[kOperands] = {B.t2h, B.t2ha, B.ovk, B.ovka, B.xv, B.xva, B.t2p, B.t2pa};
-  const long    rows[kOperands] = {v * v * o, v * v * o, o * v * o, o * v * o,
-                                   v * v * o, v * v * o, o * o * v, o * o * v};
+  // row coordinates of each array (0 occupied, 1 virtual), slowest first
+  const int types[kOperands][3] = {{1, 1, 0}, {1, 1, 0}, {0, 1, 0}, {0, 1, 0},
+                                   {1, 1, 0}, {1, 1, 0}, {0, 0, 1}, {0, 0, 1}};
+  int *      dto = nullptr, *dtv = nullptr;
+  short*     er  = nullptr;
+  hipError_t err = hipSuccess;
+  auto       done = [&](hipError_t e) {
+    if(dto) (void) hipFree(dto);
+    if(dtv) (void) hipFree(dtv);
+    if(er) (void) hipFree(er);
+    return (int) e;
+  };
+  if((err = hipMalloc((void**) &dto, sizeof(int) * o)) || (err = hipMalloc((void**) &dtv, sizeof(int) * v)))
+    return done(err);
+  if((err = hipMemcpyAsync(dto, tile_o, sizeof(int) * o, hipMemcpyHostToDevice, st)) ||
+     (err = hipMemcpyAsync(dtv, tile_v, sizeof(int) * v, hipMemcpyHostToDevice, st)))
+    return done(err);
   {
     const size_t zb = 4 * (size_t) (ko > kv ? ko : kv);
-    hipError_t   err = hipMalloc((void**) &S.zero, zb);
-    if(err) return (int) err;
-    if((err = hipMemsetAsync(S.zero, 0, zb, st))) return (int) err;
+    if((err = hipMalloc((void**) &S.zero, zb))) return done(err);
+    if((err = hipMemsetAsync(S.zero, 0, zb, st))) return done(err);
   }
+  const long rows_max = v * v * o > o * o * v ? v * v * o : o * o * v;
+  if((err = hipMalloc((void**) &er, sizeof(short) * rows_max))) return done(err);
   for(int j = 0; j < kOperands; j++) {
+    const int* const tmap[2] = {dto, dtv};
+    const long       dim[2]  = {o, v};
+    const int        nt[2]   = {nto, ntv};
+    const int*       ty      = types[j];
+    BlockMap         bm;
+    bm.D2 = dim[ty[1]];
+    bm.D3 = dim[ty[2]];
+    bm.n2 = nt[ty[1]];
+    bm.n3 = nt[ty[2]];
+    bm.t1 = tmap[ty[0]];
+    bm.t2 = tmap[ty[1]];
+    bm.t3 = tmap[ty[2]];
+    const long rows   = dim[ty[0]] * dim[ty[1]] * dim[ty[2]];
+    const long blocks = (long) nt[ty[0]] * nt[ty[1]] * nt[ty[2]];
     const long kin = j < 4 ? op : vp, kout = j < 4 ? ko : kv;
-    S.rows[j]      = rows[j];
-    hipError_t err = hipMalloc((void**) &S.e[j], sizeof(short) * rows[j]);
-    if(err) return (int) err;
-    err = hipMalloc((void**) &S.v[j], 4 * rows[j] * kout);
-    if(err) return (int) err;
-    hipLaunchKernelGGL(k_row_exp, dim3((unsigned) ((rows[j] + 3) / 4)), dim3(256), 0, st, src[j],
-                       rows[j], kin, S.e[j]);
-    const long n = rows[j] * kout;
-    hipLaunchKernelGGL(k_split, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, st, src[j],
-                       rows[j], kin, kout, S.e[j], reinterpret_cast<_Float16*>(S.v[j]));
-    if((err = hipGetLastError())) return (int) err;
+    S.rows[j]      = rows;
+    if((err = hipMalloc((void**) &S.eblk[j], sizeof(int) * blocks))) return done(err);
+    if((err = hipMalloc((void**) &S.v[j], 4 * rows * kout))) return done(err);
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned) ((blocks + 255) / 256)), dim3(256), 0, st,
+                       S.eblk[j], blocks, kNoRow);
+    hipLaunchKernelGGL(k_row_exp, dim3((unsigned) ((rows + 3) / 4)), dim3(256), 0, st, src[j], rows,
+                       kin, er);
+    hipLaunchKernelGGL(k_block_exp, dim3((unsigned) ((rows + 255) / 256)), dim3(256), 0, st, er,
+                       rows, bm, S.eblk[j]);
+    const long n = rows * kout;
+    hipLaunchKernelGGL(k_split, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, st, src[j], rows,
+                       kin, kout, bm, S.eblk[j], reinterpret_cast<_Float16*>(S.v[j]));
+    if((err = hipGetLastError())) return done(err);
   }
-  return (int) hipStreamSynchronize(st);
+  return done(hipStreamSynchronize(st));
 }
 
 int launch_spatial_mixed(const MixedDev& M, const RunDev& R, int grid, void* stream) {
