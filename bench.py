@@ -55,6 +55,18 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def profiled_traffic(config, O_, V_, precision):
+    """HBM bytes per launch of this kernel and workload from the committed
+    rocprofv3 summary (two --pmc passes, calibrated), or (None, None)."""
+    import re
+    kern = {"mixed": "k_spatial_mixed", "fp64": "k_spatial_fp64"}[precision]
+    path = os.path.join(ROOT, "profiles", f"{kern}_{config}_O{O_}_V{V_}.md")
+    if not os.path.exists(path):
+        return None, None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", open(path).read())
+    return (float(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
+
+
 def cpu_baseline(tiling, s, per_task, budget_s):
     """Times the oracle on tasks of the workload, from the 25th flop
     percentile upwards, until the budget is spent."""
@@ -92,7 +104,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
-    ap.add_argument("--schedule", default="analytic", choices=["driver", "analytic", "learned"])
+    ap.add_argument("--schedule", default="learned", choices=["driver", "analytic", "learned"])
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default ticket group shape instead of tuning it")
+    ap.add_argument("--tune-sample", type=int, default=256,
+                    help="tasks timed per candidate by the autotuner")
+    ap.add_argument("--learn-sample", type=int, default=256,
+                    help="tasks timed per precision to train the cost model")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of the graph")
     ap.add_argument("--task-sample", type=int, default=0,
                     help="profiling aid: time only this many tasks (not the benchmark workload)")
@@ -129,6 +147,30 @@ def main():
     t0 = time.time()
     ctx.prepare(prec)
     operand_s = time.time() - t0
+    # one-time calibration (untimed, like the upload): tile-shape autotuning
+    # and the learned cost model
+    tune = {}
+    t0 = time.time()
+    if not args.no_autotune:
+        for name, p in (("fp64", xt.PREC_FP64), (args.precision, prec)):
+            if name in tune:
+                continue
+            if p != prec:
+                ctx.prepare(p)
+            best, times, cands = ctx.autotune(precision=p, n_sample=args.tune_sample)
+            tune[name] = {"shape": cands[best], "candidates": cands,
+                          "sample_ms": [round(x, 2) for x in times]}
+            log(f"autotune {name}: shape {cands[best]} ({min(times):.1f} ms vs default "
+                f"{times[0]:.1f} ms on {args.tune_sample} tasks)")
+    tune_s = time.time() - t0
+    learn = None
+    t0 = time.time()
+    if args.schedule == "learned":
+        model = xt.CostModel(0, seed=1)
+        learn = ctx.learn_schedule(model, prec, n_sample=args.learn_sample)
+        model.close()
+        log(f"learned schedule: {learn}")
+    learn_s = time.time() - t0
     which = None
     if args.task_sample:
         rng = np.random.default_rng(1)
@@ -159,6 +201,8 @@ def main():
     kernel_s = float(np.mean(kms)) / 1e3
     achieved = total_flops / kernel_s / 1e12
     peak = PEAK_TFLOPS[args.precision]
+    traffic, traffic_src = (None, None) if which is not None else profiled_traffic(
+        args.config, O_, V_, args.precision)
     line = {
         "metric": "time-to-E(T) + MFMA TFLOP/s at O=48,V=384; |dE(T)| vs FP64 oracle",
         "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -174,9 +218,12 @@ def main():
                    "schedule": args.schedule, "graph": bool(graph),
                    "parallelism": "single GPU, persistent kernel over per-XCD work queues"},
         "E4": res["e4"], "E5": res["e5"], "tflops": achieved,
-        "setup_s": {"upload_and_layout": upload_s, "operand_split": operand_s},
+        "setup_s": {"upload_and_layout": upload_s, "operand_split": operand_s,
+                    "autotune": tune_s, "learn_schedule": learn_s},
+        "autotune": tune or None,
+        "learned_schedule": learn,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
                      "flops_per_launch": total_flops, "flop_count": "reference analytic count",
                      "kernel_ms": kernel_s * 1e3},
     }

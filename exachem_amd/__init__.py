@@ -82,6 +82,9 @@ def lib():
     L.xt_run_tasks.argtypes = [vp, C.POINTER(XtOpts), C.c_long, lp, dp, dp, C.POINTER(C.c_float)]
     L.xt_task_codes.argtypes = [vp, C.c_long, C.POINTER(C.c_uint64)]
     L.xt_prepare.argtypes = [vp, C.c_int]
+    L.xt_set_group_shape.argtypes = [vp, C.c_int, ip]
+    L.xt_get_group_shape.argtypes = [vp, C.c_int, ip]
+    L.xt_autotune.argtypes = [vp, C.POINTER(XtOpts), C.c_long, ip, dp, ip]
     fp = C.POINTER(C.c_float)
     L.xt_mlp_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
     L.xt_mlp_destroy.argtypes = [vp]
@@ -101,7 +104,8 @@ def lib():
                  "xt_run", "xt_last_kernel_ms", "xt_fully_fused_ccsd_t_gpu", "xt_run_tasks",
                  "xt_task_codes", "xt_prepare", "xt_mlp_create", "xt_mlp_destroy",
                  "xt_mlp_get_params", "xt_mlp_set_params", "xt_mlp_loss_grad", "xt_mlp_train",
-                 "xt_mlp_predict", "xt_task_features", "xt_set_task_cost", "xt_learn_schedule"):
+                 "xt_mlp_predict", "xt_task_features", "xt_set_task_cost", "xt_learn_schedule",
+                 "xt_set_group_shape", "xt_get_group_shape", "xt_autotune"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

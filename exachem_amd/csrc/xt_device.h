@@ -13,15 +13,9 @@ struct TaskDev {
   int       dim[6];      // tile extents
   int       off[6];      // first spatial orbital of each tile (occ or virt range)
   int       tile[6];     // spatial tile index (occupied or virtual tile list) per position
-  int       nsub[6];     // ceil(dim/4): 4-wide sub-tiles per position
-  int       nbox[6];     // ceil(nsub/2): 2^6-sub-tile boxes per position
+  int       nsub[6];     // ceil(dim/4): 4^6 sub-tiles of the FP64 kernel per position
+  int       nsubm[6];    // mixed kernel (4,4,4 | 8,8,8) boxes: ceil(dim/4), ceil(dim/8)
   int       n_sub;       // product of nsub
-  int       pad_;
-  long long n_tickets;   // 64 * prod(nbox): queue tickets of this task (some empty)
-  // mixed-precision kernel: (4,4,4 | 8,8,8) boxes
-  int       nsubm[6];    // ceil(dim/4) for holes, ceil(dim/8) for particles
-  int       nboxm[6];    // ceil(nsubm/2)
-  long long n_ticketsm;  // 64 * prod(nboxm)
   // Per output bipartition g = 3*hs + ps (hs: single hole position 0..2,
   // ps: single particle position 0..2), 6 bits each:
   //   bit0 enabled, bit1 negative sign, bit2 a1, bit3 a2, bit4 b1, bit5 b2
@@ -89,11 +83,15 @@ struct MixedDev {
 };
 
 // Work queues: one per XCD.  Queue q holds the tasks qtask[qbeg[q]..qbeg[q+1])
-// in processing order; entry e owns tickets [qtick[e], qtick[e] + n_tickets)
-// of its queue.  A ticket is one 4^6 sub-tile; 64 consecutive tickets form a
-// 2^6 box of neighbouring sub-tiles so that the workgroups resident on one XCD
-// share operand panels in its L2.
+// in processing order; entry e owns a contiguous ticket range of its queue
+// starting at qtick[e].  A ticket is one box of the kernel (one 4^6 sub-tile
+// for FP64, one (4,4,4|8,8,8) box for mixed).  64 consecutive tickets form a
+// group of neighbouring boxes, 2^glog[p] along position p (sum of glog = 6):
+// the group shape decides which operand panels the workgroups resident on one
+// XCD share in its L2.  A task has 64 * prod(ceil(nsub[p] / 2^glog[p]))
+// tickets; tickets outside the task's box grid are skipped.
 constexpr int kQueues = 8;
+constexpr int kGroupLog2 = 6;  // 64 boxes per ticket group
 
 struct RunDev {
   const TaskDev*      tasks;
@@ -107,6 +105,7 @@ struct RunDev {
   unsigned long long* cost;    // [n_tasks] summed workgroup wall-clock ticks (optional)
   unsigned long long* prog;    // [kQueues] host-mapped mirror of the ticket counters (optional)
   int                 n_tasks;
+  int                 glog[6];  // ticket group shape (log2 extent per position, sum 6)
 };
 
 // Kernel launchers (defined in xt_kernels.hip).  Return hipError_t as int.

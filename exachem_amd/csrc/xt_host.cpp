@@ -275,15 +275,7 @@ int build_plan(Plan& P, int O, int V, const std::vector<int>& occ, const std::ve
     int nd1, nd2, ns1;
     slot_codes(sp, T.d1code, T.d2code, T.s1code, nd1, nd2, ns1);
     T.factor     = P.factors[k];
-    T.n_tickets  = 64;
-    T.n_ticketsm = 64;
-    for(int p = 0; p < 6; p++) {
-      T.nbox[p]  = (T.nsub[p] + 1) / 2;
-      T.nsubm[p] = (T.dim[p] + (p < 3 ? 3 : 7)) / (p < 3 ? 4 : 8);
-      T.nboxm[p] = (T.nsubm[p] + 1) / 2;
-      T.n_tickets *= T.nbox[p];
-      T.n_ticketsm *= T.nboxm[p];
-    }
+    for(int p = 0; p < 6; p++) T.nsubm[p] = (T.dim[p] + (p < 3 ? 3 : 7)) / (p < 3 ? 4 : 8);
     double prod = 1;
     for(int p = 0; p < 6; p++) prod *= T.dim[p];
     // helper_calculate_num_ops (fused_common.hpp:131-261): 2*prod per s1 slot,
@@ -331,6 +323,9 @@ struct xt_ctx {
   hipEvent_t          ev0 = nullptr, ev1 = nullptr;
   double              last_ms = 0.0;
   std::vector<double> learned_cost;  // per-task predicted cost (learned schedule)
+  // ticket group shapes (log2 extents per position), set by the autotuner
+  int glog_fp64[6]  = {1, 1, 1, 1, 1, 1};
+  int glog_mixed[6] = {1, 1, 1, 1, 1, 1};
   // Captured launch sequence (zero fills, persistent kernel, reductions).  It
   // reads the queues from the device buffers above, so one instantiation
   // serves every schedule and task subset of the same precision / grid /
@@ -636,8 +631,19 @@ struct Queues {
   std::vector<long long> tick, total;
 };
 
+// Tickets of task T under group shape glog (one group of 64 boxes per
+// ceil(n/2^glog) cell of the task's box grid).
+static long long task_tickets(const xt::TaskDev& T, const int* glog, bool mixed) {
+  long long n = 1ll << xt::kGroupLog2;
+  for(int p = 0; p < 6; p++) {
+    const int b = mixed ? T.nsubm[p] : T.nsub[p];
+    n *= (b + (1 << glog[p]) - 1) >> glog[p];
+  }
+  return n;
+}
+
 static Queues build_queues(const Plan& P, const std::vector<int>& sel, int schedule,
-                           const std::vector<double>* cost, bool mixed) {
+                           const std::vector<double>* cost, bool mixed, const int* glog) {
   const int                     nq = xt::kQueues;
   std::vector<std::vector<int>> per(nq);
   if(schedule == XT_SCHED_DRIVER) {
@@ -661,7 +667,7 @@ static Queues build_queues(const Plan& P, const std::vector<int>& sel, int sched
     for(int t: per[q]) {
       Q.task.push_back(t);
       Q.tick.push_back(tk);
-      tk += mixed ? P.dev[t].n_ticketsm : P.dev[t].n_tickets;
+      tk += task_tickets(P.dev[t], glog, mixed);
     }
     Q.beg.push_back((int) Q.task.size());
     Q.total.push_back(tk);
@@ -715,7 +721,8 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
     (o.schedule == XT_SCHED_LEARNED && (long) c->learned_cost.size() == nt) ? &c->learned_cost
                                                                             : nullptr;
   const bool  mixed = o.precision == XT_PREC_MIXED;
-  Queues      Q     = build_queues(P, sel, o.schedule, learned, mixed);
+  const int*  glog  = mixed ? c->glog_mixed : c->glog_fp64;
+  Queues      Q     = build_queues(P, sel, o.schedule, learned, mixed, glog);
   hipStream_t st = c->stream;
   if(!Q.task.empty()) {
     XT_HIP(hipMemcpyAsync(c->d_qtask, Q.task.data(), sizeof(int) * Q.task.size(),
@@ -756,6 +763,7 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
   R.prog    = c->d_prog;
   for(int q = 0; q < xt::kQueues; q++) __atomic_store_n(&c->h_prog[q], 0ull, __ATOMIC_RELAXED);
   R.n_tasks = (int) nt;
+  for(int p = 0; p < 6; p++) R.glog[p] = glog[p];
   long long tickets = 0;
   for(long long t: Q.total) tickets += t;
   int grid = o.workgroups > 0 ? o.workgroups : c->n_cu * 2;
@@ -957,6 +965,88 @@ int xt_last_kernel_ms(xt_ctx* c, double* ms) {
   if(!c || !ms) return fail(XT_EINVAL, "xt_last_kernel_ms: null argument");
   *ms = c->last_ms;
   return XT_OK;
+}
+
+// ----------------------------------------------------------- autotuner
+namespace {
+const int kTuneShapes[XT_TUNE_CANDIDATES][6] = {
+  {1, 1, 1, 1, 1, 1},  // two boxes along every position
+  {0, 0, 0, 2, 2, 2},  // 4x4x4 over the particles
+  {2, 2, 2, 0, 0, 0},  // 4x4x4 over the holes
+  {0, 1, 1, 1, 1, 2},
+  {1, 1, 1, 0, 1, 2},
+  {0, 0, 2, 2, 2, 0},
+};
+
+bool valid_shape(const int* g) {
+  int sum = 0;
+  for(int p = 0; p < 6; p++) {
+    if(g[p] < 0 || g[p] > xt::kGroupLog2) return false;
+    sum += g[p];
+  }
+  return sum == xt::kGroupLog2;
+}
+}  // namespace
+
+int xt_set_group_shape(xt_ctx* c, int precision, const int* glog) {
+  if(!c || !glog) return fail(XT_EINVAL, "xt_set_group_shape: null argument");
+  if(precision != XT_PREC_FP64 && precision != XT_PREC_MIXED)
+    return fail(XT_EINVAL, "xt_set_group_shape: unknown precision");
+  if(!valid_shape(glog)) return fail(XT_EINVAL, "xt_set_group_shape: extents must sum to 2^6");
+  int* dst = precision == XT_PREC_MIXED ? c->glog_mixed : c->glog_fp64;
+  std::copy(glog, glog + 6, dst);
+  c->drop_graph();  // the captured launch holds the shape
+  return XT_OK;
+}
+
+int xt_get_group_shape(xt_ctx* c, int precision, int* glog) {
+  if(!c || !glog) return fail(XT_EINVAL, "xt_get_group_shape: null argument");
+  if(precision != XT_PREC_FP64 && precision != XT_PREC_MIXED)
+    return fail(XT_EINVAL, "xt_get_group_shape: unknown precision");
+  const int* src = precision == XT_PREC_MIXED ? c->glog_mixed : c->glog_fp64;
+  std::copy(src, src + 6, glog);
+  return XT_OK;
+}
+
+int xt_autotune(xt_ctx* c, const xt_opts* opts, long n_sample, int* best, double* times_ms,
+                int* shapes) {
+  if(!c || !opts || n_sample <= 0) return fail(XT_EINVAL, "xt_autotune: bad argument");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_autotune: no tiling");
+  const long n = (long) c->plan.dev.size();
+  if(n == 0) return fail(XT_ESTATE, "xt_autotune: no tasks");
+  // tasks spread evenly over the flop-sorted list
+  std::vector<long> order(n);
+  std::iota(order.begin(), order.end(), 0L);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](long a, long b) { return c->plan.dev[a].flops < c->plan.dev[b].flops; });
+  const long        ns = std::min(n_sample, n);
+  std::vector<long> pick(ns);
+  for(long i = 0; i < ns; i++) pick[i] = order[ns == 1 ? n / 2 : (i * (n - 1)) / (ns - 1)];
+  std::sort(pick.begin(), pick.end());
+  pick.erase(std::unique(pick.begin(), pick.end()), pick.end());
+  xt_opts o          = *opts;
+  o.record_task_time = 0;
+  o.progress_ms      = 0;
+  const int prec     = o.precision;
+  int       rc       = XT_OK;
+  std::vector<double> e45(2), part(2 * pick.size());
+  double              best_ms = 0;
+  int                 best_i  = -1;
+  for(int k = -1; k < XT_TUNE_CANDIDATES; k++) {  // k = -1: warm-up run
+    if((rc = xt_set_group_shape(c, prec, kTuneShapes[k < 0 ? 0 : k]))) return rc;
+    if((rc = xt_run_tasks(c, &o, (long) pick.size(), pick.data(), e45.data(), part.data(), nullptr)))
+      return rc;
+    if(k < 0) continue;
+    const double ms = c->last_ms;
+    if(times_ms) times_ms[k] = ms;
+    if(shapes) std::copy(kTuneShapes[k], kTuneShapes[k] + 6, shapes + 6 * k);
+    if(best_i < 0 || ms < best_ms) {
+      best_ms = ms;
+      best_i  = k;
+    }
+  }
+  if(best) *best = best_i;
+  return xt_set_group_shape(c, prec, kTuneShapes[best_i]);
 }
 
 // ----------------------------------------------------- learned schedule

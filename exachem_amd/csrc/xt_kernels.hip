@@ -396,16 +396,19 @@ __global__ void __launch_bounds__(NT) k_spatial_fp64(SpatialDev S, RunDev R) {
       const int ti = s_task;
       if(ti < 0) break;
       const TaskDev& T = R.tasks[ti];
-      // ticket -> sub-tile: low 6 bits = position inside the 2^6 box
-      long long box = s_local >> 6;
-      const int inb = (int) (s_local & 63);
+      // ticket -> sub-tile: the low bits select the box inside its group
+      long long box = s_local >> kGroupLog2;
+      const int inb = (int) (s_local & ((1 << kGroupLog2) - 1));
       bool      ok  = true;
       int       sub[6];
+      int       shift = 0;
 #pragma unroll
       for(int p = 0; p < 6; p++) {
-        const int nb = T.nbox[p];
-        sub[p]       = 2 * (int) (box % nb) + ((inb >> p) & 1);
+        const int gl = R.glog[p];
+        const int nb = (T.nsub[p] + (1 << gl) - 1) >> gl;
+        sub[p]       = ((int) (box % nb) << gl) + ((inb >> shift) & ((1 << gl) - 1));
         box /= nb;
+        shift += gl;
         ok = ok && sub[p] < T.nsub[p];
       }
       if(!ok) continue;

@@ -283,6 +283,27 @@ class CCSDTriples:
                                       lr, C.byref(st)))
         return {k: getattr(st, k) for k, _ in LearnStats._fields_}
 
+    def set_group_shape(self, precision, glog):
+        """Ticket group shape (log2 extents along h1,h2,h3,p4,p5,p6, sum 6)."""
+        g = np.ascontiguousarray(glog, dtype=np.int32)
+        check(lib().xt_set_group_shape(self._ctx, precision, _ip(g)))
+
+    def group_shape(self, precision):
+        g = np.zeros(6, dtype=np.int32)
+        check(lib().xt_get_group_shape(self._ctx, precision, _ip(g)))
+        return g.tolist()
+
+    def autotune(self, precision=PREC_MIXED, n_sample=256, schedule=SCHED_DRIVER, workgroups=0):
+        """Times every candidate group shape on a task sample and installs the
+        fastest; returns (best index, times in ms, candidate shapes)."""
+        o = default_opts(precision=precision, schedule=schedule, workgroups=workgroups)
+        best = np.zeros(1, dtype=np.int32)
+        times = np.zeros(6)
+        shapes = np.zeros(36, dtype=np.int32)
+        check(lib().xt_autotune(self._ctx, C.byref(o), n_sample, _ip(best), _dp(times),
+                                _ip(shapes)))
+        return int(best[0]), times.tolist(), shapes.reshape(6, 6).tolist()
+
     def task_codes(self, task):
         """(d1code, d2code, s1code) bit fields of one task."""
         codes = (C.c_uint64 * 3)()

@@ -188,6 +188,29 @@ int xt_task_codes(xt_ctx* ctx, long task, uint64_t* codes);
 int xt_last_kernel_ms(xt_ctx* ctx, double* ms);
 
 /* ------------------------------------------------------------------------
+ * Tile-shape autotuner (no counterpart in the reference, whose GPU kernel
+ * runs one 4^6 block per workgroup in a fixed order, nontc:2801-2919).
+ *
+ * Each kernel works on boxes of t3 (4^6 sub-tiles in FP64, (4,4,4|8,8,8)
+ * boxes in mixed precision); the work queue hands them out in groups of 64
+ * neighbouring boxes, 2^glog[p] along position p of (h1,h2,h3,p4,p5,p6).
+ * The group shape decides which operand panels the workgroups of one XCD
+ * share in its L2.  It never changes the energies (bit for bit).
+ * ------------------------------------------------------------------------ */
+#define XT_TUNE_CANDIDATES 6
+
+/* glog[6]: non-negative, summing to 6.  Default {1,1,1,1,1,1}. */
+int xt_set_group_shape(xt_ctx* ctx, int precision, const int* glog);
+int xt_get_group_shape(xt_ctx* ctx, int precision, int* glog);
+/* Times every candidate group shape (XT_TUNE_CANDIDATES of them) on
+ * n_sample tasks spread over the flop range at opts->precision, installs the
+ * fastest for that precision and returns its index in *best; times_ms
+ * (optional, XT_TUNE_CANDIDATES entries) receives each candidate's kernel
+ * time and shapes (optional, 6 * XT_TUNE_CANDIDATES) the candidates. */
+int xt_autotune(xt_ctx* ctx, const xt_opts* opts, long n_sample, int* best, double* times_ms,
+                int* shapes);
+
+/* ------------------------------------------------------------------------
  * Learned task-cost model for the work-queue scheduler (no counterpart in
  * the reference, whose tasks are handed out in driver order by NXTVAL,
  * ccsd_t_fused_driver.hpp:381).

@@ -217,3 +217,30 @@ def test_graph_replays_match_eager(xt):
     s = ctx.run_tasks(which, use_graph=1)
     assert np.array_equal(s["task_partials"], eager[xt.PREC_FP64]["task_partials"][which])
     ctx.close()
+
+
+def test_group_shapes_and_autotune_keep_bits(xt):
+    """Every ticket-group shape gives the same bits in both precisions; the
+    autotuner installs one of its candidates."""
+    case = next(c for c in small_cases() if c["name"] == "config0")
+    ctx, _ = load_case(xt, case)
+    shapes = [[1, 1, 1, 1, 1, 1], [0, 0, 0, 2, 2, 2], [2, 2, 2, 0, 0, 0], [0, 0, 0, 0, 0, 6],
+              [6, 0, 0, 0, 0, 0]]
+    for p in (xt.PREC_FP64, xt.PREC_MIXED):
+        ref = None
+        for g in shapes:
+            ctx.set_group_shape(p, g)
+            assert ctx.group_shape(p) == g
+            r = ctx.run(precision=p, use_graph=1)
+            print(f"[gpu] precision {p} shape {g}: E4 {r['e4']!r}", flush=True)
+            if ref is None:
+                ref = r
+            assert r["e4"] == ref["e4"] and r["e5"] == ref["e5"], (p, g)
+            assert np.array_equal(r["task_partials"], ref["task_partials"])
+        best, times, cands = ctx.autotune(precision=p, n_sample=8)
+        print(f"[gpu] autotune precision {p}: best {best} {cands[best]} times {times}", flush=True)
+        assert 0 <= best < len(cands) and ctx.group_shape(p) == cands[best]
+        assert min(times) == times[best]
+        r = ctx.run(precision=p)
+        assert r["e4"] == ref["e4"] and r["e5"] == ref["e5"]
+    ctx.close()

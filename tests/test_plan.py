@@ -157,3 +157,19 @@ def test_error_codes_not_exits(xt):
     rc = L.xt_fully_fused_ccsd_t_gpu(None, 1, 4, 4, 4, 4, 4, 4, *[None] * 6, *[zp] * 6,
                                      2, 1, 1, 2, 1, 1, 1, 1, *[one] * 6, one, None)
     assert rc == xt.XT_EINVAL
+
+
+def test_group_shape_validation(xt):
+    """Ticket group shapes are log2 extents summing to 6; others are rejected
+    with an error code, not a crash (host-only context)."""
+    ctx = host_ctx(xt, 7, 22, [3, 4], [5, 9, 8])
+    assert ctx.group_shape(xt.PREC_MIXED) == [1, 1, 1, 1, 1, 1]
+    ctx.set_group_shape(xt.PREC_MIXED, [0, 0, 0, 2, 2, 2])
+    assert ctx.group_shape(xt.PREC_MIXED) == [0, 0, 0, 2, 2, 2]
+    assert ctx.group_shape(xt.PREC_FP64) == [1, 1, 1, 1, 1, 1]
+    for bad in ([1, 1, 1, 1, 1, 2], [0, 0, 0, 0, 0, 0], [-1, 1, 1, 1, 2, 2], [7, 0, 0, 0, 0, -1]):
+        with pytest.raises(xt.XtError):
+            ctx.set_group_shape(xt.PREC_FP64, bad)
+    with pytest.raises(xt.XtError):
+        ctx.set_group_shape(5, [1, 1, 1, 1, 1, 1])
+    ctx.close()
