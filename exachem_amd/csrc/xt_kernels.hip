@@ -512,28 +512,33 @@ static int transpose(const double* in, double* out, long rows, long cols, long l
 // out[r][k] = in[r][k] for k < K, 0 for K <= k < Kp
 __global__ void k_pad_rows(double* __restrict__ out, const double* __restrict__ in, long rows,
                            long K, long Kp) {
-  const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= rows * Kp) return;
-  const long r = i / Kp, k = i % Kp;
-  out[i]       = k < K ? in[r * K + k] : 0.0;
+  const long n = rows * Kp;
+  for(long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long) gridDim.x * blockDim.x) {
+    const long r = i / Kp, k = i % Kp;
+    out[i]       = k < K ? in[r * K + k] : 0.0;
+  }
 }
 
 // out[i0][i1][i2][k] = in[i0][i1][i2][k] - in[j0][j1][j2][k] with (j0,j1,j2) =
 // (i1,i0,i2) (swap02 = 0) or (i2,i1,i0) (swap02 = 1); padding stays zero.
 __global__ void k_antisym(double* __restrict__ out, const double* __restrict__ in, long n0,
                           long n1, long n2, long Kp, int swap02) {
-  const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n0 * n1 * n2 * Kp) return;
-  long       r  = i / Kp;
-  const long k  = i % Kp;
-  const long i2 = r % n2;
-  r /= n2;
-  const long i1 = r % n1, i0 = r / n1;
-  const long j  = swap02 ? ((i2 * n1 + i1) * n2 + i0) * Kp + k : ((i1 * n1 + i0) * n2 + i2) * Kp + k;
-  out[i]        = in[i] - in[j];
+  const long n = n0 * n1 * n2 * Kp;
+  for(long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long) gridDim.x * blockDim.x) {
+    long       r  = i / Kp;
+    const long k  = i % Kp;
+    const long i2 = r % n2;
+    r /= n2;
+    const long i1 = r % n1, i0 = r / n1;
+    const long j  = swap02 ? ((i2 * n1 + i1) * n2 + i0) * Kp + k : ((i1 * n1 + i0) * n2 + i2) * Kp + k;
+    out[i]        = in[i] - in[j];
+  }
 }
 
-static unsigned nblk(long n) { return (unsigned) ((n + 255) / 256); }
+// Blocks of 256 for n elements of a grid-stride kernel.  A dispatch's grid
+// size is a 32-bit count of work-items, so the grid is capped (2^20 blocks
+// keep every CU busy) and the kernels loop over the rest.
+static unsigned nblk(long n) { return (unsigned) std::min<long>((n + 255) / 256, 1l << 20); }
 
 int build_operands(int O, int V, double* t2, double* ovoo, double*& vvvo, SpatialBufs& B,
                    void* stream) {

@@ -675,15 +675,16 @@ __global__ void k_block_exp(const short* __restrict__ er, long rows, BlockMap bm
 // Split: y = x * 2^e[block(row)] = hi + lo (FP16), written per 8 k as [hi8][lo8].
 __global__ void k_split(const double* __restrict__ x, long rows, long Kin, long Kout, BlockMap bm,
                         const int* __restrict__ eb, _Float16* __restrict__ out) {
-  const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= rows * Kout) return;
-  const long     row = i / Kout, k = i % Kout;
-  const double   y   = k < Kin ? ldexp(x[row * Kin + k], eb[bm(row)]) : 0.0;
-  const _Float16 h   = (_Float16) y;
-  const _Float16 l   = (_Float16) (y - (double) h);
-  const long     base = row * Kout * 2 + (k / 8) * 16 + (k % 8);
-  out[base]     = h;
-  out[base + 8] = l;
+  const long n = rows * Kout;
+  for(long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long) gridDim.x * blockDim.x) {
+    const long     row  = i / Kout, k = i % Kout;
+    const double   y    = k < Kin ? ldexp(x[row * Kin + k], eb[bm(row)]) : 0.0;
+    const _Float16 h    = (_Float16) y;
+    const _Float16 l    = (_Float16) (y - (double) h);
+    const long     base = row * Kout * 2 + (k / 8) * 16 + (k % 8);
+    out[base]           = h;
+    out[base + 8]       = l;
+  }
 }
 
 }  // namespace
@@ -717,6 +718,8 @@ int build_split(const SpatialBufs& B, int O, int V, const int* tile_o, int nto, 
     if((err = hipMemsetAsync(S.zero, 0, zb, st))) return done(err);
   }
   const long rows_max = v * v * o > o * o * v ? v * v * o : o * o * v;
+  // k_row_exp runs one wave per row: 64 * rows work-items in one dispatch
+  if(rows_max * 64 > (long) UINT32_MAX) return done(hipErrorInvalidConfiguration);
   if((err = hipMalloc((void**) &er, sizeof(short) * rows_max))) return done(err);
   for(int j = 0; j < kOperands; j++) {
     const int* const tmap[2] = {dto, dtv};
@@ -743,8 +746,9 @@ int build_split(const SpatialBufs& B, int O, int V, const int* tile_o, int nto, 
                        kin, er);
     hipLaunchKernelGGL(k_block_exp, dim3((unsigned) ((rows + 255) / 256)), dim3(256), 0, st, er,
                        rows, bm, S.eblk[j]);
+    // grid-stride: a dispatch's grid size is a 32-bit count of work-items
     const long n = rows * kout;
-    hipLaunchKernelGGL(k_split, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, st, src[j], rows,
+    hipLaunchKernelGGL(k_split, dim3((unsigned) std::min<long>((n + 255) / 256, 1l << 20)), dim3(256), 0, st, src[j], rows,
                        kin, kout, bm, S.eblk[j], reinterpret_cast<_Float16*>(S.v[j]));
     if((err = hipGetLastError())) return done(err);
   }
