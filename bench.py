@@ -119,6 +119,10 @@ def main():
     ap.add_argument("--fp64-full", action="store_true",
                     help="compare against a full FP64 run instead of a task sample")
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--sigma", type=float, default=0.05,
+                    help="standard deviation of the synthetic integrals (E4 scales as sigma^4)")
+    ap.add_argument("--t1-sigma", type=float, default=1e-3,
+                    help="standard deviation of the synthetic T1 amplitudes")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -136,7 +140,7 @@ def main():
     tiling = make_tiling(xt, args.config)
     O_, V_ = tiling.O, tiling.V
     t0 = time.time()
-    s = xt.synth_closed_shell(O_, V_, SEED)
+    s = xt.synth_closed_shell(O_, V_, SEED, args.sigma, args.t1_sigma)
     ctx = xt.CCSDTriples(0)
     ctx.set_tiling(tiling)
     ctx.upload(s)
@@ -216,6 +220,7 @@ def main():
                    "tasks": n_tasks if which is None else int(len(which)),
                    "task_sample": bool(which is not None), "precision": args.precision,
                    "schedule": args.schedule, "graph": bool(graph),
+                   "sigma": args.sigma, "t1_sigma": args.t1_sigma,
                    "parallelism": "single GPU, persistent kernel over per-XCD work queues"},
         "E4": res["e4"], "E5": res["e5"], "tflops": achieved,
         "setup_s": {"upload_and_layout": upload_s, "operand_split": operand_s,

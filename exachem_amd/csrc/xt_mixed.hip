@@ -227,47 +227,65 @@ __device__ __forceinline__ void read_frags(const MSmem& sm, int buf, Frags& F) {
   }
 }
 
-// 32x128x16 of this wave: three passes over its 4 accumulators; column
-// tiles outside the task tile (cmask bit clear) are skipped.
-__device__ __forceinline__ void mfma_chunk(const Slot& s, const Frags& F, unsigned cmask,
-                                           f16v (&acc)[4]) {
+// 32x128x16 of this wave: three passes over its 4 accumulators.  Column
+// tiles outside the task tile are not skipped: their B rows are the zero row,
+// and a branch per MFMA costs more issue time than the skipped MFMA.
+__device__ __forceinline__ void mfma_chunk(const Slot& s, const Frags& F, f16v (&acc)[4]) {
   const h8 ahi = as_h8(s.a[0]), alo = as_h8(s.a[1]);
 #pragma unroll
-  for(int j = 0; j < 4; j++)
-    if(cmask & (1u << j)) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, F.bhi[j], acc[j], 0, 0, 0);
+  for(int j = 0; j < 4; j++) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, F.bhi[j], acc[j], 0, 0, 0);
 #pragma unroll
-  for(int j = 0; j < 4; j++)
-    if(cmask & (1u << j)) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, F.blo[j], acc[j], 0, 0, 0);
+  for(int j = 0; j < 4; j++) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, F.blo[j], acc[j], 0, 0, 0);
 #pragma unroll
-  for(int j = 0; j < 4; j++)
-    if(cmask & (1u << j)) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, F.bhi[j], acc[j], 0, 0, 0);
+  for(int j = 0; j < 4; j++) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, F.bhi[j], acc[j], 0, 0, 0);
 }
 
-// Adds the scaled accumulators of bipartition g into the box and clears them.
-// D layout of v_mfma_f32_32x32x16_f16: register q of lane l holds row
-// 8*(q/4) + 4*(l/32) + q%4, column l%32.  The t3 index of (q, j) is a
-// per-lane base plus a q part plus a j part (disjoint bit fields), each
-// computed once per flush; the part's scale is one number.
-__device__ __forceinline__ void flush(MSmem& sm, int g, float sc, unsigned cmask, f16v (&acc)[4]) {
-  const int hs = g / 3, ps = g % 3;
-  const int hA = pair_a(hs), hB = pair_b(hs), pA = pair_a(ps), pB = pair_b(ps);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int shs = sh(hs), spa = sh(3 + pA), sps = sh(3 + ps);
-  // m = 32 wave + 8 (q/4) + 4 (lane/32) + q%4: hs = m & 3, pA = (m >> 2) & 7, pB = m >> 5
-  // n = 32 j + lane%32:                       hA = n & 3, hB = (n >> 2) & 3, ps = n >> 4
-  const int base = ((lane >> 5) << spa) | (wave << sh(3 + pB)) | ((lane & 3) << sh(hA)) |
+// Adds the scaled accumulators of a bipartition with row hole HS into the
+// box and clears them.  D layout of v_mfma_f32_32x32x16_f16: register q of
+// lane l holds row 8*(q/4) + 4*(l/32) + q%4, column l%32.  With
+//   m = 32 wave + 8 (q/4) + 4 (lane/32) + q%4: hs = m & 3, pA = (m >> 2) & 7, pB = m >> 5
+//   n = 32 j + lane%32:                       hA = n & 3, hB = (n >> 2) & 3, ps = n >> 4
+// the t3 index of (q, j) is a per-lane base plus a j part plus a q/4 part plus
+// (q%4) << sh(HS).  For HS = 0 (h1, the fastest t3 index) the four q%4 are
+// consecutive floats and go through one 16-byte LDS read and write.
+template<int HS>
+__device__ __forceinline__ void flush_h(MSmem& sm, int ps, float sc, f16v (&acc)[4]) {
+  constexpr int hA = pair_a(HS), hB = pair_b(HS), shs = sh(HS);
+  const int     pA = pair_a(ps), pB = pair_b(ps);
+  const int     lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int     spa = sh(3 + pA), sps = sh(3 + ps);
+  const int     base = ((lane >> 5) << spa) | (wave << sh(3 + pB)) | ((lane & 3) << sh(hA)) |
                    (((lane >> 2) & 3) << sh(hB)) | (((lane >> 4) & 1) << sps);
-  int joff[4];
 #pragma unroll
-  for(int j = 0; j < 4; j++) joff[j] = base + ((2 * j) << sps);
-#pragma unroll
-  for(int q = 0; q < 16; q++) {
-    const int qoff = ((q & 3) << shs) | ((2 * (q >> 2)) << spa);
+  for(int qq = 0; qq < 4; qq++)
 #pragma unroll
     for(int j = 0; j < 4; j++) {
-      if(cmask & (1u << j)) sm.t3[joff[j] + qoff] += acc[j][q] * sc;
-      acc[j][q] = 0.f;
+      const int o = base + ((2 * j) << sps) + ((2 * qq) << spa);
+      if constexpr(HS == 0) {
+        float4* p = reinterpret_cast<float4*>(&sm.t3[o]);  // o is a multiple of 4
+        float4  v = *p;
+        v.x += acc[j][4 * qq + 0] * sc;
+        v.y += acc[j][4 * qq + 1] * sc;
+        v.z += acc[j][4 * qq + 2] * sc;
+        v.w += acc[j][4 * qq + 3] * sc;
+        *p = v;
+      }
+      else {
+#pragma unroll
+        for(int r = 0; r < 4; r++) sm.t3[o + (r << shs)] += acc[j][4 * qq + r] * sc;
+      }
     }
+#pragma unroll
+  for(int j = 0; j < 4; j++)
+#pragma unroll
+    for(int q = 0; q < 16; q++) acc[j][q] = 0.f;
+}
+
+__device__ __forceinline__ void flush(MSmem& sm, int g, float sc, f16v (&acc)[4]) {
+  switch(g / 3) {
+    case 0: flush_h<0>(sm, g % 3, sc, acc); break;
+    case 1: flush_h<1>(sm, g % 3, sc, acc); break;
+    default: flush_h<2>(sm, g % 3, sc, acc); break;
   }
 }
 
@@ -280,15 +298,11 @@ __device__ __forceinline__ unsigned part_code(uint64_t d1, uint64_t d2, int gp) 
   return (unsigned) (c >> (6 * (gp >> 1))) & 63u;
 }
 
-// Column tiles of bipartition g holding a particle ps index inside the tile
-// (tile j covers ps = 2j, 2j+1), or none when this wave's row tile (pB = wave)
-// lies outside.
+// Whether this wave's row tile (pB = wave) of bipartition g lies inside the
+// task tile; a wave outside it has only zero rows and skips its flushes.
 __device__ __forceinline__ unsigned tile_mask(const MSmem& sm, int g) {
   const int ps = g % 3, pB = pair_b(ps), wave = threadIdx.x >> 6;
-  if(8 * sm.sub[3 + pB] + wave >= sm.dim[3 + pB]) return 0u;
-  const int left = sm.dim[3 + ps] - 8 * sm.sub[3 + ps];  // valid ps in this box (>= 1)
-  const int nj   = left >= 8 ? 4 : (left + 1) >> 1;
-  return (1u << nj) - 1u;
+  return 8 * sm.sub[3 + pB] + wave < sm.dim[3 + pB] ? 1u : 0u;
 }
 
 __device__ __forceinline__ void box_contract(MSmem& sm, const MixedDev& M, uint64_t d1, uint64_t d2) {
@@ -347,10 +361,10 @@ __device__ __forceinline__ void box_contract(MSmem& sm, const MixedDev& M, uint6
 #pragma unroll
     for(int s = 0; s < PF; s++) {
       read_frags(sm, (s + 1) & 1, F[(s + 1) & 1]);
-      mfma_chunk(ring[s], F[s & 1], cmask, acc);
+      mfma_chunk(ring[s], F[s & 1], acc);
       stage(sm, ring[(s + 2) % PF], s & 1);
       if(++cch == cn) {
-        if(cmask) flush(sm, cgp >> 1, ring[s].sc, cmask, acc);
+        if(cmask) flush(sm, cgp >> 1, ring[s].sc, acc);
         cch = 0;
         do {
           ++cgp;
