@@ -69,6 +69,8 @@ struct SplitBufs {
   int*           eblk[kOperands];  // [blocks] exponents, row-coordinate tile order
   long           rows[kOperands];
   unsigned char* zero;             // zero row, 4 * max(Kp16) bytes
+  float*         pscale;           // [task][18] contraction part scales (0 for parts off)
+  long           pscale_n;         // tasks pscale was built for
 };
 struct MixedDev {
   int                  O, V, Ko, Kv;  // Ko = roundup(O,16), Kv = roundup(V,16)
@@ -80,6 +82,7 @@ struct MixedDev {
   const unsigned char* v[kOperands];
   const int*           eblk[kOperands];
   const unsigned char* zero;  // zero row (4 * max(Ko, Kv) bytes): rows outside a tile read it
+  const float*         pscale;  // [task][18] part scales (build_part_scales)
 };
 
 // Work queues: one per XCD.  Queue q holds the tasks qtask[qbeg[q]..qbeg[q+1])
@@ -123,6 +126,11 @@ int launch_zero_u64(unsigned long long* p, long n, void* stream);
 int build_split(const SpatialBufs& B, int O, int V, const int* tile_o, int nto, const int* tile_v,
                 int ntv, SplitBufs& S, void* stream);
 int launch_spatial_mixed(const MixedDev& M, const RunDev& R, int grid, void* stream);
+// Scale of every contraction part of every task (sign and term coefficients
+// times 2^-(eA + eB) of the blocks the part reads), from the split's block
+// exponents: out[18 * task + gp].
+int build_part_scales(const TaskDev* tasks, int n_tasks, const MixedDev& M, float* out,
+                      void* stream);
 
 // Task cost model (xt_mlp.hip): 7 inputs (6 tile extents / 48, precision
 // flag) -> 64 -> 64 -> 1, FP32.

@@ -369,6 +369,9 @@ struct xt_ctx {
     }
     if(split.zero) (void) hipFree(split.zero);
     split.zero = nullptr;
+    if(split.pscale) (void) hipFree(split.pscale);
+    split.pscale   = nullptr;
+    split.pscale_n = 0;
     have_split = false;
   }
   void free_plan_dev() {
@@ -600,6 +603,29 @@ void xt_default_opts(xt_opts* o) {
   o->workgroups       = 0;
 }
 
+// Device view of the mixed form's operands (the split must exist).
+static xt::MixedDev mixed_dev(const xt_ctx* c) {
+  const Plan&  P = c->plan;
+  xt::MixedDev M = {};
+  M.O            = P.O;
+  M.V            = P.V;
+  M.Ko           = (P.O + 15) / 16 * 16;
+  M.Kv           = (P.V + 15) / 16 * 16;
+  M.eo           = c->in.eo;
+  M.ev           = c->in.ev;
+  M.t1           = c->in.t1;
+  M.vvoo         = c->in.vvoo;
+  for(int j = 0; j < xt::kOperands; j++) {
+    M.v[j]    = c->split.v[j];
+    M.eblk[j] = c->split.eblk[j];
+  }
+  M.zero   = c->split.zero;
+  M.pscale = c->split.pscale;
+  M.nto    = P.noab / 2;
+  M.ntv    = P.nvab / 2;
+  return M;
+}
+
 static int ensure_plan_dev(xt_ctx* c) {
   const Plan& P  = c->plan;
   const long  nt = std::max<long>((long) P.dev.size(), 1);
@@ -770,21 +796,7 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
   grid     = (int) std::max<long long>(1, std::min<long long>(grid, tickets));
   xt::MixedDev M = {};
   if(mixed) {
-    M.O    = P.O;
-    M.V    = P.V;
-    M.Ko   = (P.O + 15) / 16 * 16;
-    M.Kv   = (P.V + 15) / 16 * 16;
-    M.eo   = c->in.eo;
-    M.ev   = c->in.ev;
-    M.t1   = c->in.t1;
-    M.vvoo = c->in.vvoo;
-    for(int j = 0; j < xt::kOperands; j++) {
-      M.v[j] = c->split.v[j];
-      M.eblk[j] = c->split.eblk[j];
-    }
-    M.zero = c->split.zero;
-    M.nto  = c->plan.noab / 2;
-    M.ntv  = c->plan.nvab / 2;
+    M    = mixed_dev(c);
     grid = o.workgroups > 0 ? o.workgroups : c->n_cu;
     grid = (int) std::max<long long>(1, std::min<long long>(grid, tickets));
   }
@@ -875,7 +887,21 @@ static int check_run_state(xt_ctx* c, const xt_opts& o) {
   if(o.schedule < XT_SCHED_DRIVER || o.schedule > XT_SCHED_LEARNED)
     return fail(XT_EINVAL, "xt_run: bad schedule");
   XT_HIP(hipSetDevice(c->device));
-  return ensure_plan_dev(c);
+  int rc = ensure_plan_dev(c);
+  if(rc) return rc;
+  const long nt = (long) c->plan.dev.size();
+  if(o.precision == XT_PREC_MIXED && c->split.pscale_n != nt) {
+    if(c->split.pscale) (void) hipFree(c->split.pscale);
+    c->split.pscale   = nullptr;
+    c->split.pscale_n = 0;
+    XT_HIP(hipMalloc((void**) &c->split.pscale, sizeof(float) * 18 * std::max<long>(nt, 1)));
+    const xt::MixedDev M = mixed_dev(c);
+    int err = xt::build_part_scales(c->d_tasks, (int) nt, M, c->split.pscale, c->stream);
+    if(!err) err = (int) hipStreamSynchronize(c->stream);
+    if(err) return fail(XT_EHIP, std::string("part scales: ") + hipGetErrorString((hipError_t) err));
+    c->split.pscale_n = nt;
+  }
+  return XT_OK;
 }
 
 int xt_prepare(xt_ctx* c, int precision) {

@@ -67,6 +67,7 @@ struct __align__(16) MSmem {
   } u;
   double eps[36];  // h1,h2,h3 (4 each) at 0,4,8; p4,p5,p6 (8 each) at 12,20,28
   double red[2 * NW];
+  float  scale[18];  // scale of each contraction part of the box's task
   int    sub[6], dim[6], off[6], tile[6];
 };
 
@@ -124,6 +125,18 @@ struct Slot {
   float sc;    // scale of the chunk's part
 };
 
+// Operand array, row and coefficient of the term c1*x - c2*y of a
+// spin-orbital element (bits bx, by of the part code): both terms -> the
+// antisymmetrised array (holding x - y), one term -> the plain array.
+__host__ __device__ __forceinline__ void choose_term(unsigned c, unsigned bx, unsigned by, int both,
+                                                     int single, int rx, int ry, int& arr, int& row,
+                                                     float& coef) {
+  const bool hx = (c & bx) != 0, hy = (c & by) != 0;
+  arr  = hx && hy ? both : single;
+  row  = hx ? rx : ry;
+  coef = (hx || hy) ? (hx ? 1.f : -1.f) : 0.f;
+}
+
 // Part gp = 2*g + (d2 ? 1 : 0) of bipartition g for this thread; gp = 18 is
 // the zero part that pads the chunk count to a multiple of the ring depth.
 __device__ __forceinline__ Part part_setup(const MSmem& sm, const MixedDev& M, int gp,
@@ -144,22 +157,11 @@ __device__ __forceinline__ Part part_setup(const MSmem& sm, const MixedDev& M, i
   pair_of(hs, hA, hB);
   pair_of(ps, pA, pB);
   const int  O = M.O, V = M.V;      // row indices stay below 2^31 (V*V*O rows at most)
-  const int  no = M.nto, nv = M.ntv;
   const long K = d2 ? M.Kv : M.Ko;  // padded row length (elements)
   // operand array, row, block and coefficient of c1*x - c2*y (both terms ->
   // the antisymmetrised array, which holds x - y)
-  auto choose = [](unsigned c, unsigned bx, unsigned by, int both, int single, int rx, int ry,
-                   int kx, int ky, int& arr, int& row, int& blk, float& coef) {
-    const bool hx = (c & bx) != 0, hy = (c & by) != 0;
-    arr  = hx && hy ? both : single;
-    row  = hx ? rx : ry;
-    blk  = hx ? kx : ky;
-    coef = (hx || hy) ? (hx ? 1.f : -1.f) : 0.f;
-  };
-  const int tHS = sm.tile[hs], tHA = sm.tile[hA], tHB = sm.tile[hB];
-  const int tPS = sm.tile[3 + ps], tPA = sm.tile[3 + pA], tPB = sm.tile[3 + pB];
   P.nch = (int) (K / KC);
-  int   aa, ra, ka, ab, rb, kb;
+  int   aa, ra, ab, rb;
   float ca, cb;
   {
     const int  m   = 32 * wave + (lane & 31);
@@ -168,12 +170,10 @@ __device__ __forceinline__ Part part_setup(const MSmem& sm, const MixedDev& M, i
     const bool mok = gHS < sm.dim[hs] && gPA < sm.dim[3 + pA] && gPB < sm.dim[3 + pB];
     const int HS = sm.off[hs] + (mok ? gHS : 0), PA = sm.off[3 + pA] + (mok ? gPA : 0),
               PB = sm.off[3 + pB] + (mok ? gPB : 0);
-    if(!d2)  // t2h[a][b][i]: rows (pA,pB,hs) or (pB,pA,hs), tiles (V,V,O)
-      choose(code, 4, 8, 1, 0, (PA * V + PB) * O + HS, (PB * V + PA) * O + HS,
-             (tPA * nv + tPB) * no + tHS, (tPB * nv + tPA) * no + tHS, aa, ra, ka, ca);
-    else  // xv[c][b][i]: rows (pB,pA,hs) or (pA,pB,hs), tiles (V,V,O)
-      choose(code, 4, 8, 5, 4, (PB * V + PA) * O + HS, (PA * V + PB) * O + HS,
-             (tPB * nv + tPA) * no + tHS, (tPA * nv + tPB) * no + tHS, aa, ra, ka, ca);
+    if(!d2)  // t2h[a][b][i]: rows (pA,pB,hs) or (pB,pA,hs)
+      choose_term(code, 4, 8, 1, 0, (PA * V + PB) * O + HS, (PB * V + PA) * O + HS, aa, ra, ca);
+    else  // xv[c][b][i]: rows (pB,pA,hs) or (pA,pB,hs)
+      choose_term(code, 4, 8, 5, 4, (PB * V + PA) * O + HS, (PA * V + PB) * O + HS, aa, ra, ca);
     P.pa = (mok && ca != 0.f) ? M.v[aa] + (long) ra * K * 4 + 32 * (lane >> 5)
                               : M.zero + 32 * (lane >> 5);
   }
@@ -183,18 +183,13 @@ __device__ __forceinline__ Part part_setup(const MSmem& sm, const MixedDev& M, i
     const bool nok = gHA < sm.dim[hA] && gHB < sm.dim[hB] && gPS < sm.dim[3 + ps];
     const int HA = sm.off[hA] + (nok ? gHA : 0), HB = sm.off[hB] + (nok ? gHB : 0),
               PS = sm.off[3 + ps] + (nok ? gPS : 0);
-    if(!d2)  // ovk[i][a][j]: rows (hB,ps,hA) or (hA,ps,hB), tiles (O,V,O)
-      choose(code, 16, 32, 3, 2, (HB * V + PS) * O + HA, (HA * V + PS) * O + HB,
-             (tHB * nv + tPS) * no + tHA, (tHA * nv + tPS) * no + tHB, ab, rb, kb, cb);
-    else  // t2p[i][j][a]: rows (hA,hB,ps) or (hB,hA,ps), tiles (O,O,V)
-      choose(code, 16, 32, 7, 6, (HA * O + HB) * V + PS, (HB * O + HA) * V + PS,
-             (tHA * no + tHB) * nv + tPS, (tHB * no + tHA) * nv + tPS, ab, rb, kb, cb);
+    if(!d2)  // ovk[i][a][j]: rows (hB,ps,hA) or (hA,ps,hB)
+      choose_term(code, 16, 32, 3, 2, (HB * V + PS) * O + HA, (HA * V + PS) * O + HB, ab, rb, cb);
+    else  // t2p[i][j][a]: rows (hA,hB,ps) or (hB,hA,ps)
+      choose_term(code, 16, 32, 7, 6, (HA * O + HB) * V + PS, (HB * O + HA) * V + PS, ab, rb, cb);
     P.pb = (nok && cb != 0.f) ? M.v[ab] + (long) rb * K * 4 + 16 * piece : M.zero + 16 * piece;
   }
-  // block exponents: the same address in every lane (scalar loads)
-  const int   ea = M.eblk[aa][ka], eb = M.eblk[ab][kb];
-  const float sg = (code & 2) ? -1.f : 1.f;
-  P.sc           = sg * ca * cb * ldexpf(1.f, -(ea + eb));
+  P.sc = sm.scale[gp];  // per task, copied to LDS at the box start
   return P;
 }
 
@@ -613,6 +608,7 @@ __global__ void __launch_bounds__(NT) k_spatial_mixed(MixedDev M, RunDev R) {
           sm.off[p]  = T.off[p];
           sm.tile[p] = T.tile[p];
         }
+      if(tid >= 64 && tid < 64 + 18) sm.scale[tid - 64] = M.pscale[18 * (size_t) ti + tid - 64];
       {
         float4* t4 = reinterpret_cast<float4*>(sm.t3);
         for(int e = tid; e < BOX / 4; e += NT) t4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -701,7 +697,48 @@ __global__ void k_split(const double* __restrict__ x, long rows, long Kin, long 
   }
 }
 
+// One thread per (task, part): the part's sign and term coefficients times
+// 2^-(eA + eB), eA and eB the exponents of the (tile, tile, tile) blocks of
+// the two operands (the same for every row of the part).
+__global__ void k_part_scales(const TaskDev* __restrict__ tasks, int n_tasks, MixedDev M,
+                              float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= 18 * n_tasks) return;
+  const TaskDev& T  = tasks[i / 18];
+  const int      gp = i % 18;
+  if(!part_on(T.d1code, T.d2code, gp)) {
+    out[i] = 0.f;
+    return;
+  }
+  const unsigned code = part_code(T.d1code, T.d2code, gp);
+  const int      g = gp >> 1, d2 = gp & 1, hs = g / 3, ps = g % 3;
+  const int      hA = pair_a(hs), hB = pair_b(hs), pA = pair_a(ps), pB = pair_b(ps);
+  const int      no = M.nto, nv = M.ntv;
+  const int      tHS = T.tile[hs], tHA = T.tile[hA], tHB = T.tile[hB];
+  const int      tPS = T.tile[3 + ps], tPA = T.tile[3 + pA], tPB = T.tile[3 + pB];
+  int            aa, ka, ab, kb;
+  float          ca, cb;
+  if(!d2) {  // t2h blocks (V,V,O): (pA,pB,hs) / (pB,pA,hs); ovk (O,V,O): (hB,ps,hA) / (hA,ps,hB)
+    choose_term(code, 4, 8, 1, 0, (tPA * nv + tPB) * no + tHS, (tPB * nv + tPA) * no + tHS, aa, ka, ca);
+    choose_term(code, 16, 32, 3, 2, (tHB * nv + tPS) * no + tHA, (tHA * nv + tPS) * no + tHB, ab, kb, cb);
+  }
+  else {  // xv blocks (V,V,O): (pB,pA,hs) / (pA,pB,hs); t2p (O,O,V): (hA,hB,ps) / (hB,hA,ps)
+    choose_term(code, 4, 8, 5, 4, (tPB * nv + tPA) * no + tHS, (tPA * nv + tPB) * no + tHS, aa, ka, ca);
+    choose_term(code, 16, 32, 7, 6, (tHA * no + tHB) * nv + tPS, (tHB * no + tHA) * nv + tPS, ab, kb, cb);
+  }
+  const float sg = (code & 2) ? -1.f : 1.f;
+  out[i]         = sg * ca * cb * ldexpf(1.f, -(M.eblk[aa][ka] + M.eblk[ab][kb]));
+}
+
 }  // namespace
+
+int build_part_scales(const TaskDev* tasks, int n_tasks, const MixedDev& M, float* out,
+                      void* stream) {
+  if(n_tasks <= 0) return 0;
+  hipLaunchKernelGGL(k_part_scales, dim3((unsigned) ((18l * n_tasks + 255) / 256)), dim3(256), 0,
+                     (hipStream_t) stream, tasks, n_tasks, M, out);
+  return (int) hipGetLastError();
+}
 
 int build_split(const SpatialBufs& B, int O, int V, const int* tile_o, int nto, const int* tile_v,
                 int ntv, SplitBufs& S, void* stream) {
