@@ -48,6 +48,9 @@ constexpr int KC  = 16;       // K per chunk (one MFMA K step)
 constexpr int BOX = 32768;
 constexpr int PF  = 4;        // ring depth: chunks whose loads are in flight per thread
 constexpr int S1P = 3;        // s1 slots staged per sweep
+// the unrolled ring loop picks stage and fragment buffers by s & 1, which is
+// the chunk's parity only when the ring depth is even
+static_assert(PF % 2 == 0, "ring depth must be even");
 
 struct __align__(16) MSmem {
   float t3[BOX];
@@ -161,35 +164,39 @@ __device__ __forceinline__ Part part_setup(const MSmem& sm, const MixedDev& M, i
   // operand array, row, block and coefficient of c1*x - c2*y (both terms ->
   // the antisymmetrised array, which holds x - y)
   P.nch = (int) (K / KC);
+  // the box's tile data of the six positions, read together (one LDS wait);
+  // the row tests below use & so that no read sits behind a branch
+  const int sHS = sm.sub[hs], sPA = sm.sub[3 + pA], sPB = sm.sub[3 + pB];
+  const int sHA = sm.sub[hA], sHB = sm.sub[hB], sPS = sm.sub[3 + ps];
+  const int dHS = sm.dim[hs], dPA = sm.dim[3 + pA], dPB = sm.dim[3 + pB];
+  const int dHA = sm.dim[hA], dHB = sm.dim[hB], dPS = sm.dim[3 + ps];
+  const int oHS = sm.off[hs], oPA = sm.off[3 + pA], oPB = sm.off[3 + pB];
+  const int oHA = sm.off[hA], oHB = sm.off[hB], oPS = sm.off[3 + ps];
+  P.sc          = sm.scale[gp];  // per task, copied to LDS at the box start
   int   aa, ra, ab, rb;
   float ca, cb;
   {
     const int  m   = 32 * wave + (lane & 31);
-    const int  gHS = 4 * sm.sub[hs] + (m & 3), gPA = 8 * sm.sub[3 + pA] + ((m >> 2) & 7),
-               gPB = 8 * sm.sub[3 + pB] + (m >> 5);
-    const bool mok = gHS < sm.dim[hs] && gPA < sm.dim[3 + pA] && gPB < sm.dim[3 + pB];
-    const int HS = sm.off[hs] + (mok ? gHS : 0), PA = sm.off[3 + pA] + (mok ? gPA : 0),
-              PB = sm.off[3 + pB] + (mok ? gPB : 0);
+    const int  gHS = 4 * sHS + (m & 3), gPA = 8 * sPA + ((m >> 2) & 7), gPB = 8 * sPB + (m >> 5);
+    const bool mok = (gHS < dHS) & (gPA < dPA) & (gPB < dPB);
+    const int  HS = oHS + (mok ? gHS : 0), PA = oPA + (mok ? gPA : 0), PB = oPB + (mok ? gPB : 0);
     if(!d2)  // t2h[a][b][i]: rows (pA,pB,hs) or (pB,pA,hs)
       choose_term(code, 4, 8, 1, 0, (PA * V + PB) * O + HS, (PB * V + PA) * O + HS, aa, ra, ca);
     else  // xv[c][b][i]: rows (pB,pA,hs) or (pA,pB,hs)
       choose_term(code, 4, 8, 5, 4, (PB * V + PA) * O + HS, (PA * V + PB) * O + HS, aa, ra, ca);
-    P.pa = (mok && ca != 0.f) ? M.v[aa] + (long) ra * K * 4 + 32 * (lane >> 5)
-                              : M.zero + 32 * (lane >> 5);
+    P.pa = (mok & (ca != 0.f)) ? M.v[aa] + (long) ra * K * 4 + 32 * (lane >> 5)
+                               : M.zero + 32 * (lane >> 5);
   }
   {
-    const int  gHA = 4 * sm.sub[hA] + (n & 3), gHB = 4 * sm.sub[hB] + ((n >> 2) & 3),
-               gPS = 8 * sm.sub[3 + ps] + (n >> 4);
-    const bool nok = gHA < sm.dim[hA] && gHB < sm.dim[hB] && gPS < sm.dim[3 + ps];
-    const int HA = sm.off[hA] + (nok ? gHA : 0), HB = sm.off[hB] + (nok ? gHB : 0),
-              PS = sm.off[3 + ps] + (nok ? gPS : 0);
+    const int  gHA = 4 * sHA + (n & 3), gHB = 4 * sHB + ((n >> 2) & 3), gPS = 8 * sPS + (n >> 4);
+    const bool nok = (gHA < dHA) & (gHB < dHB) & (gPS < dPS);
+    const int  HA = oHA + (nok ? gHA : 0), HB = oHB + (nok ? gHB : 0), PS = oPS + (nok ? gPS : 0);
     if(!d2)  // ovk[i][a][j]: rows (hB,ps,hA) or (hA,ps,hB)
       choose_term(code, 16, 32, 3, 2, (HB * V + PS) * O + HA, (HA * V + PS) * O + HB, ab, rb, cb);
     else  // t2p[i][j][a]: rows (hA,hB,ps) or (hB,hA,ps)
       choose_term(code, 16, 32, 7, 6, (HA * O + HB) * V + PS, (HB * O + HA) * V + PS, ab, rb, cb);
-    P.pb = (nok && cb != 0.f) ? M.v[ab] + (long) rb * K * 4 + 16 * piece : M.zero + 16 * piece;
+    P.pb = (nok & (cb != 0.f)) ? M.v[ab] + (long) rb * K * 4 + 16 * piece : M.zero + 16 * piece;
   }
-  P.sc = sm.scale[gp];  // per task, copied to LDS at the box start
   return P;
 }
 
