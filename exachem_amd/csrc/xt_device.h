@@ -125,6 +125,38 @@ int launch_zero_u64(unsigned long long* p, long n, void* stream);
 // tile_o[O] / tile_v[V]: spatial tile index of every orbital (host arrays).
 int build_split(const SpatialBufs& B, int O, int V, const int* tile_o, int nto, const int* tile_v,
                 int ntv, SplitBufs& S, void* stream);
+// Device gather of the per-task packed buffers (xt_gather.hip, xt_gather.cpp).
+// One GatherBlock is one Tensor::get + hptt permute of the host gathers: the
+// source block's extents, spins and spatial starts per dimension (source
+// order), the permutation (destination dimension k is source dimension
+// perm[k], row-major), the factor, and where it lands.
+enum GatherKind { kGatherT1 = 0, kGatherT2 = 1, kGatherIJAB = 2, kGatherIJKA = 3, kGatherIABC = 4 };
+struct GatherBlock {
+  long   dst;       // element offset in the destination buffer
+  long   n;         // elements
+  int    buf;       // destination: 0 d1_t2, 1 d1_v2, 2 d2_t2, 3 d2_v2, 4 s1_t1, 5 s1_v2
+  int    kind;      // GatherKind
+  int    nd;        // 2 (T1) or 4
+  int    size[4];   // source extents
+  int    start[4];  // spatial start per source dimension
+  int    spin[4];   // 0 alpha, 1 beta
+  int    perm[4];
+  double alpha;     // +1 or -1
+};
+struct GatherSrc {
+  long          O, V, Op, Vp;
+  const double* t1;    // [V][O]
+  const double* t2h;   // [V][V][O][Op]
+  const double* vvoo;  // [V][V][O][O]
+  const double* ovk;   // [O][V][O][Op]
+  const double* xv;    // [V][V][O][Vp], xv[c][b][i][a] = vvvo[a][c][b][i]
+};
+struct GatherDst {
+  double* buf[6];
+};
+int launch_gather(const GatherBlock* blocks, int n_blocks, long max_elems, const GatherSrc& S,
+                  const GatherDst& D, void* stream);
+
 int launch_spatial_mixed(const MixedDev& M, const RunDev& R, int grid, void* stream);
 // Scale of every contraction part of every task (sign and term coefficients
 // times 2^-(eA + eB) of the blocks the part reads), from the split's block

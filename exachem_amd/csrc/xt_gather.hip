@@ -1,0 +1,91 @@
+// Device-side gather of the per-task packed operand buffers: the work of the
+// reference's host gathers ccsd_t_data_s1_new (ccsd_t_all_fused_singles.hpp:34-391),
+// ccsd_t_data_d1_new (ccsd_t_all_fused_doubles1.hpp:37-307) and ccsd_t_data_d2_new
+// (ccsd_t_all_fused_doubles2.hpp:37-360), which read TAMM blocks (Tensor::get)
+// and permute them with hptt, done here from the device-resident closed-shell
+// inputs.  The host plans the list of blocks (xt_gather.cpp); this kernel
+// writes every element of every block.
+#include <hip/hip_runtime.h>
+
+#include "xt_device.h"
+
+namespace xt {
+namespace {
+
+// Spin-orbital element of one of the five tensors the gathers read, formed
+// from the spatial closed-shell inputs (spin-orbital order is [alpha, beta]
+// per space).  The combination rule c1*x - c2*y and the spatial sources:
+//   T1   t1(a,i)        = [sa=si] t1[A][I]
+//   T2   t2(a,b,i,j)    = [sa=si][sb=sj] t2[A][B][I][J] - [sa=sj][sb=si] t2[A][B][J][I]
+//   IJAB <ij||ab>       = [si=sa][sj=sb] (ai|bj) - [si=sb][sj=sa] (bi|aj)          (vvoo)
+//   IJKA <ij||ka>       = [si=sk][sj=sa] (ja|ik) - [sj=sk][si=sa] (ia|jk)          (ovoo)
+//   IABC <ia||bc>       = [si=sb][sa=sc] (ac|bi) - [si=sc][sa=sb] (ab|ci)          (vvvo)
+// with t2 read from t2h[a][b][i][Op], ovoo from ovk[i][a][j][Op] and vvvo
+// from xv[c][b][i][Vp] = vvvo[a][c][b][i].
+__device__ __forceinline__ double so_value(const GatherSrc& S, int kind, const int* sp, const long* x) {
+  const long O = S.O, V = S.V, Op = S.Op, Vp = S.Vp;
+  auto comb = [](bool c1, bool c2, double a, double b) {
+    return c1 && c2 ? a - b : (c1 ? a : (c2 ? -b : 0.0));
+  };
+  switch(kind) {
+    case kGatherT1: return sp[0] == sp[1] ? S.t1[x[0] * O + x[1]] : 0.0;
+    case kGatherT2: {
+      const bool c1 = sp[0] == sp[2] && sp[1] == sp[3], c2 = sp[0] == sp[3] && sp[1] == sp[2];
+      const long ab = (x[0] * V + x[1]) * O;
+      return comb(c1, c2, c1 ? S.t2h[(ab + x[2]) * Op + x[3]] : 0.0,
+                  c2 ? S.t2h[(ab + x[3]) * Op + x[2]] : 0.0);
+    }
+    case kGatherIJAB: {
+      const bool c1 = sp[0] == sp[2] && sp[1] == sp[3], c2 = sp[0] == sp[3] && sp[1] == sp[2];
+      return comb(c1, c2, c1 ? S.vvoo[((x[2] * V + x[3]) * O + x[0]) * O + x[1]] : 0.0,
+                  c2 ? S.vvoo[((x[3] * V + x[2]) * O + x[0]) * O + x[1]] : 0.0);
+    }
+    case kGatherIJKA: {
+      const bool c1 = sp[0] == sp[2] && sp[1] == sp[3], c2 = sp[1] == sp[2] && sp[0] == sp[3];
+      return comb(c1, c2, c1 ? S.ovk[((x[1] * V + x[3]) * O + x[0]) * Op + x[2]] : 0.0,
+                  c2 ? S.ovk[((x[0] * V + x[3]) * O + x[1]) * Op + x[2]] : 0.0);
+    }
+    default: {  // kGatherIABC
+      const bool c1 = sp[0] == sp[2] && sp[1] == sp[3], c2 = sp[0] == sp[3] && sp[1] == sp[2];
+      return comb(c1, c2, c1 ? S.xv[((x[3] * V + x[2]) * O + x[0]) * Vp + x[1]] : 0.0,
+                  c2 ? S.xv[((x[2] * V + x[3]) * O + x[0]) * Vp + x[1]] : 0.0);
+    }
+  }
+}
+
+// blockIdx.y = block; threads stride over its elements in destination order.
+// Destination element e has (row-major, last fastest) coordinates i[k] over
+// the permuted extents sizeA[perm[k]]; source coordinate perm[k] is i[k]
+// (hptt's row-major permutation), and the value is alpha times the element.
+__global__ void k_gather(const GatherBlock* __restrict__ blocks, GatherSrc S, GatherDst D) {
+  const GatherBlock& B = blocks[blockIdx.y];
+  double*            dst = D.buf[B.buf] + B.dst;
+  const int          nd = B.nd;
+  for(long e = (long) blockIdx.x * blockDim.x + threadIdx.x; e < B.n; e += (long) gridDim.x * blockDim.x) {
+    long r = e, x[4] = {0, 0, 0, 0};
+    int  sp[4] = {0, 0, 0, 0};
+    for(int k = nd - 1; k >= 0; k--) {
+      const int  d  = B.perm[k];
+      const long sz = B.size[d];
+      x[d]          = B.start[d] + r % sz;
+      sp[d]         = B.spin[d];
+      r /= sz;
+    }
+    dst[e] = B.alpha * so_value(S, B.kind, sp, x);
+  }
+}
+
+}  // namespace
+
+int launch_gather(const GatherBlock* blocks, int n_blocks, long max_elems, const GatherSrc& S,
+                  const GatherDst& D, void* stream) {
+  if(n_blocks <= 0) return 0;
+  if(n_blocks > 65535) return (int) hipErrorInvalidConfiguration;
+  const long     per = (max_elems + 255) / 256;
+  const unsigned gx  = (unsigned) (per < 512 ? (per > 0 ? per : 1) : 512);
+  hipLaunchKernelGGL(k_gather, dim3(gx, (unsigned) n_blocks), dim3(256), 0, (hipStream_t) stream,
+                     blocks, S, D);
+  return (int) hipGetLastError();
+}
+
+}  // namespace xt

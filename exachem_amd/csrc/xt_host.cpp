@@ -320,6 +320,11 @@ struct xt_ctx {
   unsigned long long* d_cost   = nullptr;
   double *            d_task_e = nullptr, *d_e45 = nullptr;
   long                dev_tasks_n = -1;
+  // device-side gather: block descriptors of the last call (device copy and
+  // the host list it was uploaded from, kept alive until the next call)
+  xt::GatherBlock*              d_gather     = nullptr;
+  long                          d_gather_cap = 0;
+  std::vector<xt::GatherBlock>  h_gather;
   hipEvent_t          ev0 = nullptr, ev1 = nullptr;
   double              last_ms = 0.0;
   std::vector<double> learned_cost;  // per-task predicted cost (learned schedule)
@@ -448,6 +453,7 @@ int xt_destroy(xt_ctx* c) {
   if(c->ev1) (void) hipEventDestroy(c->ev1);
   if(c->stream) (void) hipStreamDestroy(c->stream);
   if(c->h_prog) (void) hipHostFree(c->h_prog);
+  if(c->d_gather) (void) hipFree(c->d_gather);
   delete c;
   return XT_OK;
 }
@@ -984,6 +990,70 @@ int xt_task_codes(xt_ctx* c, long task, uint64_t* codes) {
   codes[0] = c->plan.dev[task].d1code;
   codes[1] = c->plan.dev[task].d2code;
   codes[2] = c->plan.dev[task].s1code;
+  return XT_OK;
+}
+
+// ----------------------------------------------------- device-side gather
+static xt::GatherTiling gather_tiling(const xt_ctx* c) {
+  const Plan& P = c->plan;
+  return xt::GatherTiling{P.noab, P.nvab, P.k_range.data(), P.k_spin.data(), P.spatial_off.data()};
+}
+
+int xt_packed_sizes(xt_ctx* c, long* sizes) {
+  if(!c || !sizes) return fail(XT_EINVAL, "xt_packed_sizes: null argument");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_packed_sizes: no tiling");
+  xt::packed_sizes(gather_tiling(c), sizes);
+  return XT_OK;
+}
+
+int xt_gather_task(xt_ctx* c, const long* task, void* stream, double* d1_t2, double* d1_v2,
+                   double* d2_t2, double* d2_v2, double* s1_t1, double* s1_v2, int* s1_size,
+                   int* s1_exec, int* d1_size, int* d1_exec, int* d2_size, int* d2_exec,
+                   int* counts) {
+  if(!c || !task || !d1_t2 || !d1_v2 || !d2_t2 || !d2_v2 || !s1_t1 || !s1_v2 || !s1_size ||
+     !s1_exec || !d1_size || !d1_exec || !d2_size || !d2_exec || !counts)
+    return fail(XT_EINVAL, "xt_gather_task: null argument");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_gather_task: no tiling");
+  if(c->device < 0) return fail(XT_ENODEV, "xt_gather_task: host-only context");
+  if(!c->have_inputs) return fail(XT_ESTATE, "xt_gather_task: no inputs uploaded");
+  const Plan& P = c->plan;
+  for(int k = 0; k < 6; k++) {
+    const bool occ = k < 3;
+    const long lo = occ ? 0 : P.noab, hi = occ ? P.noab : P.noab + P.nvab;
+    if(task[k] < lo || task[k] >= hi) return fail(XT_EINVAL, "xt_gather_task: tile index out of range");
+  }
+  xt::plan_gather(gather_tiling(c), P.restricted, task, c->h_gather, s1_size, s1_exec, d1_size,
+                  d1_exec, d2_size, d2_exec, counts);
+  const long n = (long) c->h_gather.size();
+  if(n == 0) return XT_OK;
+  XT_HIP(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t) stream;
+  if(n > c->d_gather_cap) {
+    // the previous descriptors may still be read by a gather on the stream
+    XT_HIP(hipStreamSynchronize(st));
+    if(c->d_gather) (void) hipFree(c->d_gather);
+    c->d_gather     = nullptr;
+    c->d_gather_cap = 0;
+    XT_HIP(hipMalloc((void**) &c->d_gather, sizeof(xt::GatherBlock) * n));
+    c->d_gather_cap = n;
+  }
+  XT_HIP(hipMemcpyAsync(c->d_gather, c->h_gather.data(), sizeof(xt::GatherBlock) * n,
+                        hipMemcpyHostToDevice, st));
+  long max_elems = 0;
+  for(const auto& B: c->h_gather) max_elems = std::max(max_elems, B.n);
+  xt::GatherSrc S;
+  S.O    = P.O;
+  S.V    = P.V;
+  S.Op   = (P.O + 7) / 8 * 8;
+  S.Vp   = (P.V + 7) / 8 * 8;
+  S.t1   = c->in.t1;
+  S.t2h  = c->in.t2h;
+  S.vvoo = c->in.vvoo;
+  S.ovk  = c->in.ovk;
+  S.xv   = c->in.xv;
+  xt::GatherDst D = {{d1_t2, d1_v2, d2_t2, d2_v2, s1_t1, s1_v2}};
+  const int     err = xt::launch_gather(c->d_gather, (int) n, max_elems, S, D, st);
+  if(err) return fail(XT_EHIP, std::string("xt_gather_task: ") + hipGetErrorString((hipError_t) err));
   return XT_OK;
 }
 

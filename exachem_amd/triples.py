@@ -304,6 +304,39 @@ class CCSDTriples:
                                 _ip(shapes)))
         return int(best[0]), times.tolist(), shapes.reshape(6, 6).tolist()
 
+    def packed_sizes(self):
+        """Packed buffer sizes of the per-task path (doubles): s1_t1, s1_v2,
+        d1_t2, d1_v2, d2_t2, d2_v2, and the slot counts max_d1, max_d2
+        (ccsd_t_fused_driver.hpp:176-195)."""
+        out = np.zeros(8, dtype=np.int64)
+        check(lib().xt_packed_sizes(self._ctx, out.ctypes.data_as(C.POINTER(C.c_long))))
+        names = ("s1_t1", "s1_v2", "d1_t2", "d1_v2", "d2_t2", "d2_v2", "max_d1", "max_d2")
+        return {n: int(v) for n, v in zip(names, out)}
+
+    def gather_task(self, tiles, bufs, stream=None):
+        """Device-side gather of one task's packed buffers, in place of
+        ccsd_t_data_{s1,d1,d2}_new (ccsd_t_all_fused.hpp:149-178).
+
+        tiles: (t_h1b, t_h2b, t_h3b, t_p4b, t_p5b, t_p6b) spin-orbital tile
+        indices; bufs: device pointers (int) keyed d1_t2, d1_v2, d2_t2, d2_v2,
+        s1_t1, s1_v2, sized by packed_sizes(); stream: hipStream_t (int) or
+        None for the null stream.  Returns (meta, counts): meta holds the
+        host int32 arrays s1_size, s1_exec, d1_size, d1_exec, d2_size,
+        d2_exec; counts the enabled s1, d1, d2 slots."""
+        t = np.ascontiguousarray(tiles, dtype=np.int64)
+        noab, nvab = 2 * len(self.tiling.occ), 2 * len(self.tiling.vir)
+        meta = {"s1_size": np.zeros(6, np.int32), "s1_exec": np.zeros(9, np.int32),
+                "d1_size": np.zeros(7 * noab, np.int32), "d1_exec": np.zeros(9 * noab, np.int32),
+                "d2_size": np.zeros(7 * nvab, np.int32), "d2_exec": np.zeros(9 * nvab, np.int32)}
+        counts = np.zeros(3, np.int32)
+        names = ("d1_t2", "d1_v2", "d2_t2", "d2_v2", "s1_t1", "s1_v2")
+        check(lib().xt_gather_task(
+            self._ctx, t.ctypes.data_as(C.POINTER(C.c_long)), stream,
+            *[C.c_void_p(int(bufs[n])) for n in names],
+            *[_ip(meta[k]) for k in ("s1_size", "s1_exec", "d1_size", "d1_exec", "d2_size",
+                                     "d2_exec")], _ip(counts)))
+        return meta, [int(x) for x in counts]
+
     def task_codes(self, task):
         """(d1code, d2code, s1code) bit fields of one task."""
         codes = (C.c_uint64 * 3)()

@@ -184,6 +184,36 @@ int xt_run_tasks(xt_ctx* ctx, const xt_opts* opts, long n, const long* which, do
  * b1, b2). */
 int xt_task_codes(xt_ctx* ctx, long task, uint64_t* codes);
 
+/* ------------------------------------ device-side gather (per-task path) */
+/*
+ * Replaces the host gathers ccsd_t_data_s1_new (ccsd_t_all_fused_singles.hpp:559-580),
+ * ccsd_t_data_d1_new (ccsd_t_all_fused_doubles1.hpp:491-514) and
+ * ccsd_t_data_d2_new (ccsd_t_all_fused_doubles2.hpp:547-570), called per task
+ * from ccsd_t_all_fused.hpp:149-178.  Instead of TAMM blocks (Tensor::get)
+ * permuted on the host by hptt, the packed buffers of one task are written
+ * on `stream` from the context's device-resident inputs (xt_upload_*), so
+ * they feed xt_fully_fused_ccsd_t_gpu without a host round trip.
+ *
+ * task = (t_h1b, t_h2b, t_h3b, t_p4b, t_p5b, t_p6b), spin-orbital tile
+ * indices of the context's tiling (occupied tiles 0..noab-1, virtual
+ * noab..noab+nvab-1).  The six device buffers are sized by xt_packed_sizes;
+ * only the blocks of enabled slots are written (slot k of a buffer starts at
+ * k * size / max_slots), as in the reference.  The host arrays are filled as
+ * the reference fills df_simple_*_size / df_simple_*_exec / df_num_*_enabled:
+ * s1_size[6], s1_exec[9], d1_size[7*noab], d1_exec[9*noab], d2_size[7*nvab],
+ * d2_exec[9*nvab] (exec -1 where no slot runs), counts[3] = enabled s1, d1, d2
+ * slots.  The context's block list is reused by the next call: issue calls
+ * of one context on one stream.
+ */
+int xt_gather_task(xt_ctx* ctx, const long task[6], void* stream, double* d1_t2, double* d1_v2,
+                   double* d2_t2, double* d2_v2, double* s1_t1, double* s1_v2, int* s1_size,
+                   int* s1_exec, int* d1_size, int* d1_exec, int* d2_size, int* d2_exec,
+                   int counts[3]);
+/* Buffer sizes of the packed path, the driver's size_T_s1_t1, size_T_s1_v2,
+ * size_T_d1_t2, size_T_d1_v2, size_T_d2_t2, size_T_d2_v2 (doubles) and
+ * max_d1_kernels_pertask, max_d2_kernels_pertask (ccsd_t_fused_driver.hpp:176-195). */
+int xt_packed_sizes(xt_ctx* ctx, long sizes[8]);
+
 /* Device time of the last xt_run's main kernel (ms), from HIP events. */
 int xt_last_kernel_ms(xt_ctx* ctx, double* ms);
 
