@@ -67,6 +67,56 @@ def profiled_traffic(config, O_, V_, precision):
     return (float(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
 
 
+def gather_leg(xt, ctx, tiling, s, n_tasks, cpu=True):
+    """The per-task path's gather (ccsd_t_data_{s1,d1,d2}_new) on the device
+    (xt_gather_task) for a seeded sample of the workload's tasks, timed with
+    HIP events on its stream; beside it the oracle's restatement of the host
+    gathers from the same spatial inputs on one host core, into buffers
+    allocated once as the driver does.  Algorithmic bytes: 8 written + at
+    least 8 read per packed double."""
+    from exachem_amd.hipmem import DevBuf, Event, Stream
+    tasks, _ = ctx.task_list()
+    rng = np.random.default_rng(7)
+    pick = np.sort(rng.choice(len(tasks), min(n_tasks, len(tasks)), replace=False))
+    sz = ctx.packed_sizes()
+    names = ("d1_t2", "d1_v2", "d2_t2", "d2_v2", "s1_t1", "s1_v2")
+    dev = {n: DevBuf(8 * sz[n]) for n in names}
+    ptrs = {n: dev[n].ptr.value for n in names}
+    st, e0, e1 = Stream(), Event(), Event()
+    ctx.gather_task(tasks[pick[0]], ptrs, st.h)  # warm-up (first launch)
+    st.sync()
+    gpu_ms, elems = 0.0, 0
+    for k in pick:
+        e0.record(st.h)
+        ctx.gather_task(tasks[k], ptrs, st.h)
+        e1.record(st.h)
+        gpu_ms += e1.elapsed_ms(e0)
+        elems += ctx.gather_last_elems()
+    for b in dev.values():
+        b.free()
+    for x in (st, e0, e1):
+        x.destroy()
+    out = {"tasks": int(len(pick)), "doubles_written": int(elems), "gpu_ms": gpu_ms,
+           "gpu_ms_per_task": gpu_ms / len(pick),
+           "roofline": {"bound": "hbm", "achieved": 16 * elems / gpu_ms / 1e6, "peak": 8000.0,
+                        "unit": "GB/s", "frac": 16 * elems / gpu_ms / 1e6 / 8000.0,
+                        "bytes_per_double": "8 written + 8 read (lower bound)"}}
+    if cpu:
+        from oracle import oracle as O
+        O.build()
+        occ, vir = tiling.occ, tiling.vir
+        g = O.SpatialGather(O.Tiling(occ, occ, vir, vir), s)
+        t0, n_cpu = time.perf_counter(), 0
+        for k in pick:
+            n_cpu += g(tasks[k])
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        out["cpu_ms"] = cpu_ms
+        out["cpu_threads"] = 1
+        out["cpu_doubles_written"] = int(n_cpu)
+        out["speedup_vs_cpu"] = cpu_ms / gpu_ms
+    return out
+
+
 def cpu_baseline(tiling, s, per_task, budget_s):
     """Times the oracle on tasks of the workload, from the 25th flop
     percentile upwards, until the budget is spent."""
@@ -123,6 +173,8 @@ def main():
                     help="standard deviation of the synthetic integrals (E4 scales as sigma^4)")
     ap.add_argument("--t1-sigma", type=float, default=1e-3,
                     help="standard deviation of the synthetic T1 amplitudes")
+    ap.add_argument("--gather-tasks", type=int, default=16,
+                    help="tasks of the device-side gather leg (0 skips it)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -263,6 +315,11 @@ def main():
         }
         log(f"fp64 {f['kernel_ms']:.1f} ms, {args.precision} {m['kernel_ms']:.1f} ms, "
             f"rel dE4 {d4 / abs(f['e4']):.2e}")
+
+    if args.gather_tasks > 0:
+        line["device_gather"] = gather_leg(xt, ctx, tiling, s, args.gather_tasks,
+                                           cpu=not args.no_cpu_baseline)
+        log(f"device gather: {line['device_gather']}")
 
     if not args.no_cpu_baseline:
         cb = cpu_baseline(tiling, s, per_task, args.cpu_budget)

@@ -99,6 +99,7 @@ def lib():
                                     C.c_float, C.POINTER(LearnStats)]
     L.xt_gather_task.argtypes = [vp, lp, vp] + [vp] * 6 + [ip] * 7
     L.xt_packed_sizes.argtypes = [vp, lp]
+    L.xt_gather_last_elems.argtypes = [vp, lp]
     L.xt_fully_fused_ccsd_t_gpu.argtypes = (
         [vp, sz] + [sz] * 6 + [vp] * 6 + [ip] * 6 + [sz] * 8 + [vp] * 6 + [vp, vp])
     for name in ("xt_create", "xt_destroy", "xt_set_tiling", "xt_num_tasks", "xt_task_list",
@@ -108,7 +109,7 @@ def lib():
                  "xt_mlp_get_params", "xt_mlp_set_params", "xt_mlp_loss_grad", "xt_mlp_train",
                  "xt_mlp_predict", "xt_task_features", "xt_set_task_cost", "xt_learn_schedule",
                  "xt_set_group_shape", "xt_get_group_shape", "xt_autotune", "xt_gather_task",
-                 "xt_packed_sizes"):
+                 "xt_packed_sizes", "xt_gather_last_elems"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

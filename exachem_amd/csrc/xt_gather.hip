@@ -61,14 +61,17 @@ __global__ void k_gather(const GatherBlock* __restrict__ blocks, GatherSrc S, Ga
   const GatherBlock& B = blocks[blockIdx.y];
   double*            dst = D.buf[B.buf] + B.dst;
   const int          nd = B.nd;
-  for(long e = (long) blockIdx.x * blockDim.x + threadIdx.x; e < B.n; e += (long) gridDim.x * blockDim.x) {
-    long r = e, x[4] = {0, 0, 0, 0};
-    int  sp[4] = {0, 0, 0, 0};
+  const unsigned     n  = (unsigned) B.n;  // a block holds fewer than 2^31 doubles (host check)
+  // 32-bit index decode: a 64-bit divide costs several times more
+  for(unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    unsigned r = e;
+    long     x[4] = {0, 0, 0, 0};
+    int      sp[4] = {0, 0, 0, 0};
     for(int k = nd - 1; k >= 0; k--) {
-      const int  d  = B.perm[k];
-      const long sz = B.size[d];
-      x[d]          = B.start[d] + r % sz;
-      sp[d]         = B.spin[d];
+      const int      d  = B.perm[k];
+      const unsigned sz = (unsigned) B.size[d];
+      x[d]              = B.start[d] + (long) (r % sz);
+      sp[d]             = B.spin[d];
       r /= sz;
     }
     dst[e] = B.alpha * so_value(S, B.kind, sp, x);

@@ -325,6 +325,7 @@ struct xt_ctx {
   xt::GatherBlock*              d_gather     = nullptr;
   long                          d_gather_cap = 0;
   std::vector<xt::GatherBlock>  h_gather;
+  long                          gather_elems = 0;  // doubles written by the last gather
   hipEvent_t          ev0 = nullptr, ev1 = nullptr;
   double              last_ms = 0.0;
   std::vector<double> learned_cost;  // per-task predicted cost (learned schedule)
@@ -1025,6 +1026,11 @@ int xt_gather_task(xt_ctx* c, const long* task, void* stream, double* d1_t2, dou
   xt::plan_gather(gather_tiling(c), P.restricted, task, c->h_gather, s1_size, s1_exec, d1_size,
                   d1_exec, d2_size, d2_exec, counts);
   const long n = (long) c->h_gather.size();
+  c->gather_elems = 0;
+  for(const auto& B: c->h_gather) {
+    if(B.n >= (1l << 31)) return fail(XT_EINVAL, "xt_gather_task: block of 2^31 or more doubles");
+    c->gather_elems += B.n;
+  }
   if(n == 0) return XT_OK;
   XT_HIP(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t) stream;
@@ -1054,6 +1060,12 @@ int xt_gather_task(xt_ctx* c, const long* task, void* stream, double* d1_t2, dou
   xt::GatherDst D = {{d1_t2, d1_v2, d2_t2, d2_v2, s1_t1, s1_v2}};
   const int     err = xt::launch_gather(c->d_gather, (int) n, max_elems, S, D, st);
   if(err) return fail(XT_EHIP, std::string("xt_gather_task: ") + hipGetErrorString((hipError_t) err));
+  return XT_OK;
+}
+
+int xt_gather_last_elems(xt_ctx* c, long* n) {
+  if(!c || !n) return fail(XT_EINVAL, "xt_gather_last_elems: null argument");
+  *n = c->gather_elems;
   return XT_OK;
 }
 

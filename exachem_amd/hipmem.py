@@ -1,6 +1,7 @@
 """Minimal ctypes access to the HIP runtime the native library links
-(/opt/rocm/lib/libamdhip64.so): device buffers for the per-task drop-in test.
-torch is not used in the GPU tests because it bundles a second HIP runtime."""
+(/opt/rocm/lib/libamdhip64.so): device buffers, streams and events for the
+per-task path (xt_gather_task, xt_fully_fused_ccsd_t_gpu), which takes device
+pointers.  torch is not used because it bundles a second HIP runtime."""
 import ctypes as C
 
 import numpy as np
@@ -21,6 +22,11 @@ def rt():
         _rt.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
         _rt.hipStreamDestroy.argtypes = [C.c_void_p]
         _rt.hipStreamSynchronize.argtypes = [C.c_void_p]
+        _rt.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        _rt.hipEventDestroy.argtypes = [C.c_void_p]
+        _rt.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        _rt.hipEventSynchronize.argtypes = [C.c_void_p]
+        _rt.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
     return _rt
 
 
@@ -67,3 +73,22 @@ class Stream:
 
     def destroy(self):
         rt().hipStreamDestroy(self.h)
+
+
+class Event:
+    def __init__(self):
+        self.h = C.c_void_p()
+        _ok(rt().hipEventCreate(C.byref(self.h)), "hipEventCreate")
+
+    def record(self, stream):
+        _ok(rt().hipEventRecord(self.h, stream), "hipEventRecord")
+
+    def elapsed_ms(self, start):
+        """Milliseconds from event `start` to this one (waits for this one)."""
+        _ok(rt().hipEventSynchronize(self.h), "hipEventSynchronize")
+        ms = C.c_float()
+        _ok(rt().hipEventElapsedTime(C.byref(ms), start.h, self.h), "hipEventElapsedTime")
+        return ms.value
+
+    def destroy(self):
+        rt().hipEventDestroy(self.h)

@@ -337,6 +337,12 @@ class CCSDTriples:
                                      "d2_exec")], _ip(counts)))
         return meta, [int(x) for x in counts]
 
+    def gather_last_elems(self):
+        """Doubles written by the last gather_task."""
+        n = C.c_long()
+        check(lib().xt_gather_last_elems(self._ctx, C.byref(n)))
+        return n.value
+
     def task_codes(self, task):
         """(d1code, d2code, s1code) bit fields of one task."""
         codes = (C.c_uint64 * 3)()

@@ -213,6 +213,8 @@ int xt_gather_task(xt_ctx* ctx, const long task[6], void* stream, double* d1_t2,
  * size_T_d1_t2, size_T_d1_v2, size_T_d2_t2, size_T_d2_v2 (doubles) and
  * max_d1_kernels_pertask, max_d2_kernels_pertask (ccsd_t_fused_driver.hpp:176-195). */
 int xt_packed_sizes(xt_ctx* ctx, long sizes[8]);
+/* Doubles written by the context's last xt_gather_task (the enabled blocks). */
+int xt_gather_last_elems(xt_ctx* ctx, long* n);
 
 /* Device time of the last xt_run's main kernel (ms), from HIP events. */
 int xt_last_kernel_ms(xt_ctx* ctx, double* ms);

@@ -1116,6 +1116,50 @@ int oracle_task_packed(int noab, int nvab, const int* tiles, const int* spins, i
   return 0;
 }
 
+// The three host gathers of one task from the spatial inputs, written into
+// caller-allocated buffers sized by oracle_buffer_sizes without clearing them
+// first, as the reference's driver reuses its buffers
+// (ccsd_t_all_fused.hpp:149-178).  Returns the doubles written (the enabled
+// blocks); meta/counts as oracle_task_packed.  For timing the host gather.
+long oracle_gather_spatial(int noab, int nvab, const int* tiles, const int* spins, int is_restricted,
+                           int O, int V, const double* t1s, const double* t2s, const double* vvoo,
+                           const double* ovoo, const double* vvvo, const long* task, double* s1_t1,
+                           double* s1_v2, double* d1_t2, double* d1_v2, double* d2_t2,
+                           double* d2_v2, int* meta, int* counts) {
+  SpatialSrc S;
+  S.O = O;
+  S.V = V;
+  S.t1s = t1s;
+  S.t2s = t2s;
+  S.vvoo = vvoo;
+  S.ovoo = ovoo;
+  S.vvvo = vvvo;
+  SOProblem P = make_problem(noab, nvab, tiles, spins, nullptr, nullptr, nullptr, nullptr, nullptr,
+                             nullptr);
+  if(P.nO != 2 * (size_t) O || P.nV != 2 * (size_t) V) return -2;
+  P.src = &S;
+  const Sizes Z = driver_sizes(P);
+  size_t      t[6];
+  for(int i = 0; i < 6; i++) t[i] = (size_t) task[i];
+  int* s1_size = meta;
+  int* s1_exec = s1_size + 6;
+  int* d1_size = s1_exec + 9;
+  int* d1_exec = d1_size + 7 * noab;
+  int* d2_size = d1_exec + 9 * noab;
+  int* d2_exec = d2_size + 7 * nvab;
+  std::fill(s1_exec, s1_exec + 9, -1);
+  std::fill(d1_exec, d1_exec + 9 * noab, -1);
+  std::fill(d2_exec, d2_exec + 9 * nvab, -1);
+  size_t elems = 0;
+  data_s1_new(is_restricted != 0, P, t[0], t[1], t[2], t[3], t[4], t[5], Z.s1_t1, Z.s1_v2, s1_size,
+              s1_exec, s1_t1, s1_v2, &counts[0], true, &elems);
+  data_d1_new(is_restricted != 0, P, t[0], t[1], t[2], t[3], t[4], t[5], Z.max_d1_kernels_pertask,
+              Z.d1_t2, Z.d1_v2, d1_t2, d1_v2, nullptr, d1_size, d1_exec, &counts[1], true, &elems);
+  data_d2_new(is_restricted != 0, P, t[0], t[1], t[2], t[3], t[4], t[5], Z.max_d2_kernels_pertask,
+              Z.d2_t2, Z.d2_v2, d2_t2, d2_v2, nullptr, d2_size, d2_exec, &counts[2], true, &elems);
+  return (long) elems;
+}
+
 int oracle_num_threads() {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -74,6 +74,10 @@ def lib():
                                          C.c_int] + [dp] * 5 + [C.c_long, lp, dp, dp]
         L.oracle_buffer_sizes.restype = None
         L.oracle_buffer_sizes.argtypes = [C.c_int, C.c_int, ip, ip, lp]
+        L.oracle_gather_spatial.restype = C.c_long
+        L.oracle_gather_spatial.argtypes = ([C.c_int, C.c_int, ip, ip, C.c_int, C.c_int, C.c_int]
+                                            + [dp] * 5 + [C.POINTER(C.c_long)] + [dp] * 6
+                                            + [ip, ip])
         L.oracle_task_packed.restype = C.c_int
         L.oracle_task_packed.argtypes = [C.c_int, C.c_int, ip, ip, C.c_int] + [dp] * 5 + [
             lp] + [dp] * 6 + [ip, ip]
@@ -261,6 +265,40 @@ def task_packed(t: Tiling, so, task, restricted=True):
         m[name] = meta[o:o + cnt].copy()
         o += cnt
     return bufs, m, tuple(int(c) for c in counts), sz
+
+
+class SpatialGather:
+    """The host gathers of the reference, restated, run from the spatial
+    inputs into buffers allocated once (as the driver reuses them), for
+    timing the host side of the per-task path."""
+
+    def __init__(self, t: Tiling, s, restricted=True):
+        self.t, self.s, self.restricted = t, s, restricted
+        self.sz = buffer_sizes(t)
+        self.bufs = {k: np.empty(self.sz[k]) for k in
+                     ["s1_t1", "s1_v2", "d1_t2", "d1_v2", "d2_t2", "d2_v2"]}
+        self.meta = np.zeros(6 + 9 + 16 * t.noab + 16 * t.nvab, dtype=np.int32)
+        self.counts = np.zeros(3, dtype=np.int32)
+
+    def __call__(self, task):
+        """Gathers one task; returns the doubles written."""
+        L = lib()
+        t, s = self.t, self.s
+        tiles, tp = _i(t.tiles)
+        spins, sp = _i(t.spins)
+        O, V = s["t1"].shape[1], s["t1"].shape[0]
+        tk, tkp = _l(np.asarray(task, dtype=np.int64))
+        n = L.oracle_gather_spatial(
+            t.noab, t.nvab, tp, sp, int(self.restricted), O, V,
+            *[np.ascontiguousarray(s[k]).ctypes.data_as(C.POINTER(C.c_double))
+              for k in ("t1", "t2", "vvoo", "ovoo", "vvvo")], tkp,
+            *[self.bufs[k].ctypes.data_as(C.POINTER(C.c_double)) for k in
+              ["s1_t1", "s1_v2", "d1_t2", "d1_v2", "d2_t2", "d2_v2"]],
+            self.meta.ctypes.data_as(C.POINTER(C.c_int)),
+            self.counts.ctypes.data_as(C.POINTER(C.c_int)))
+        if n < 0:
+            raise RuntimeError(f"oracle_gather_spatial failed ({n})")
+        return int(n)
 
 
 def num_threads():

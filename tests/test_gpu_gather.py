@@ -41,7 +41,7 @@ def test_packed_sizes_match_driver(xt, oracle):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["uneven_small", "uneven_small_unrestricted", "config0"])
 def test_device_gather_matches_host_gather(xt, oracle, name):
-    from tests.hiprt import DevBuf, Stream
+    from exachem_amd.hipmem import DevBuf, Stream
     case = next(c for c in small_cases() if c["name"] == name)
     ctx, s = load_case(xt, case)
     t, tasks, fac = _tasks_for(case, oracle)
@@ -87,3 +87,27 @@ def test_device_gather_matches_host_gather(xt, oracle, name):
         b.free()
     st.destroy()
     ctx.close()
+
+
+def test_spatial_host_gather_matches_dense(xt, oracle):
+    """CPU: the oracle's host gather from spatial inputs (the CPU side that
+    bench.py times) writes the same buffers and metadata as its dense
+    spin-orbital form, which the GPU tests above pin against the device."""
+    case = next(c for c in small_cases() if c["name"] == "uneven_small")
+    s = xt.synth_closed_shell(case["O"], case["V"], case["seed"], case["sigma"], case["t1_sigma"])
+    t, tasks, _ = _tasks_for(case, oracle)
+    so = oracle.expand_closed_shell(case["O"], case["V"], s["t1"], s["t2"], s["vvoo"], s["ovoo"],
+                                    s["vvvo"])
+    g = oracle.SpatialGather(t, s, case["restricted"])
+    for k in range(0, len(tasks), 5):
+        for b in g.bufs.values():
+            b[:] = 0.0
+        n = g(tasks[k])
+        bufs, meta, counts, _ = oracle.task_packed(t, so, tasks[k], case["restricted"])
+        assert tuple(int(c) for c in g.counts) == counts
+        assert n > 0
+        for key in bufs:
+            assert np.array_equal(g.bufs[key].view(np.uint64), bufs[key].view(np.uint64)), (k, key)
+        m = np.concatenate([meta[key] for key in ("s1_size", "s1_exec", "d1_size", "d1_exec",
+                                                  "d2_size", "d2_exec")])
+        assert np.array_equal(g.meta, m), k

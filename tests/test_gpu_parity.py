@@ -90,7 +90,7 @@ def test_packed_dropin_matches_oracle(xt, oracle):
     """xt_fully_fused_ccsd_t_gpu fed with the reference's packed buffers
     (oracle's restatement of ccsd_t_data_{s1,d1,d2}_new) gives each task's
     energy after the host-side reduction of hostEnergyReduce."""
-    from tests.hiprt import DevBuf, Stream
+    from exachem_amd.hipmem import DevBuf, Stream
     st = Stream()
     for name in ("uneven_small", "config0"):
         case = next(c for c in small_cases() if c["name"] == name)
