@@ -116,6 +116,15 @@ struct RunDev {
 // device copies t2 [V][V][O][O], ovoo [O][V][O][O] and vvvo [V][V][V][O]
 // (consumed; vvvo is freed as soon as xv exists to bound peak memory).
 // Returns 0 or a hipError_t / -1 for allocation failure.
+// vvoo [V][V][O][O], ovoo [O][V][O][O] and xv (the vvvo operand layout) from
+// Cholesky vectors L [O+V][O+V][K] on the device (xt_chol.cpp, rocBLAS).
+int cholesky_integrals(int O, int V, long K, const double* L, double* vvoo, double* ovoo,
+                       double* xv, void* stream);
+// xva = xv with the (c,b) pair antisymmetrised (xv must exist).
+int build_xva(int O, int V, SpatialBufs& B, void* stream);
+// Reorders the (ai|bj) GEMM result [(a,i)][(b,j)] into vvoo[a][b][i][j].
+int launch_vvoo_order(const double* in, double* out, int O, int V, void* stream);
+// With vvvo == nullptr the xv/xva pair is left to the caller.
 int build_operands(int O, int V, double* t2, double* ovoo, double*& vvvo, SpatialBufs& B,
                    void* stream);
 int launch_spatial_fp64(const SpatialDev& S, const RunDev& R, int grid, void* stream);

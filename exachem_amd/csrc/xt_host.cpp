@@ -576,6 +576,61 @@ static int upload_common(xt_ctx* c, const double* eo, const double* ev, const do
   return XT_OK;
 }
 
+int xt_upload_cholesky(xt_ctx* c, const double* eps_occ, const double* eps_vir, const double* t1,
+                       const double* t2, long nchol, const double* chol) {
+  if(!c || !eps_occ || !eps_vir || !t1 || !t2 || !chol || nchol <= 0)
+    return fail(XT_EINVAL, "xt_upload_cholesky: null argument or nchol <= 0");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_upload_cholesky: call xt_set_tiling first");
+  if(c->device < 0) return fail(XT_ENODEV, "xt_upload_cholesky: host-only context");
+  const size_t o = c->plan.O, v = c->plan.V, nmo = o + v;
+  if((long) (nmo * nchol) > (1l << 31) - 1 || (long) (v * o) > (1l << 31) - 1)
+    return fail(XT_EINVAL, "xt_upload_cholesky: nmo * nchol beyond the BLAS 32-bit leading dimension");
+  XT_HIP(hipSetDevice(c->device));
+  c->free_inputs();
+  hipStream_t st = c->stream;
+  double *    t2_d = nullptr, *ovoo_d = nullptr, *L_d = nullptr, *none = nullptr;
+  auto        cleanup = [&]() {
+    for(double* p: {t2_d, ovoo_d, L_d})
+      if(p) (void) hipFree(p);
+    c->free_inputs();
+  };
+  auto up = [&](double** dst, const double* src, size_t n) -> int {
+    hipError_t e = hipMalloc((void**) dst, n * sizeof(double));
+    if(e != hipSuccess) {
+      *dst = nullptr;
+      return fail(XT_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(double)) + " bytes failed");
+    }
+    if(src) e = hipMemcpyAsync(*dst, src, n * sizeof(double), hipMemcpyHostToDevice, st);
+    if(e != hipSuccess) return fail(XT_EHIP, std::string("upload copy: ") + hipGetErrorString(e));
+    return XT_OK;
+  };
+  const size_t vp = (v + 7) / 8 * 8;
+  int          rc;
+  if((rc = up(&c->in.eo, eps_occ, o)) || (rc = up(&c->in.ev, eps_vir, v)) ||
+     (rc = up(&c->in.t1, t1, v * o)) || (rc = up(&t2_d, t2, v * v * o * o)) ||
+     (rc = up(&L_d, chol, nmo * nmo * (size_t) nchol)) || (rc = up(&c->in.vvoo, nullptr, v * v * o * o)) ||
+     (rc = up(&ovoo_d, nullptr, o * v * o * o)) || (rc = up(&c->in.xv, nullptr, v * v * o * vp))) {
+    cleanup();
+    return rc;
+  }
+  int err = xt::cholesky_integrals((int) o, (int) v, nchol, L_d, c->in.vvoo, ovoo_d, c->in.xv, st);
+  if(!err) {
+    (void) hipFree(L_d);
+    L_d = nullptr;
+    err = xt::build_operands((int) o, (int) v, t2_d, ovoo_d, none, c->in, st);
+  }
+  if(!err) err = xt::build_xva((int) o, (int) v, c->in, st);
+  if(err) {
+    cleanup();
+    return fail(err == (int) hipErrorOutOfMemory ? XT_ENOMEM : XT_EHIP,
+                std::string("cholesky integrals: ") + hipGetErrorString((hipError_t) err));
+  }
+  (void) hipFree(t2_d);
+  (void) hipFree(ovoo_d);
+  c->have_inputs = true;
+  return XT_OK;
+}
+
 int xt_upload_closed_shell(xt_ctx* c, const double* eps_occ, const double* eps_vir,
                            const double* t1, const double* t2, const double* vvoo,
                            const double* ovoo, const double* vvvo) {

@@ -572,6 +572,7 @@ int build_operands(int O, int V, double* t2, double* ovoo, double*& vvvo, Spatia
                      vp, 0);
   if((e = hipStreamSynchronize(st))) return (int) e;
   (void) hipFree(tmp);
+  if(!vvvo) return 0;  // xv comes from elsewhere (build_xv_cholesky), then build_xva
   // d2 M side: xv[c][b][i][a] = vvvo[a][c][b][i], padded in a
   if((e = alloc(&B.xv, v * v * o * vp))) return (int) e;
   if((e = hipMemsetAsync(B.xv, 0, sizeof(double) * v * v * o * vp, st))) return (int) e;
@@ -579,11 +580,35 @@ int build_operands(int O, int V, double* t2, double* ovoo, double*& vvvo, Spatia
   if((e = hipStreamSynchronize(st))) return (int) e;
   (void) hipFree(vvvo);
   vvvo = nullptr;
-  if((e = alloc(&B.xva, v * v * o * vp))) return (int) e;
+  return build_xva(O, V, B, stream);
+}
+
+int build_xva(int O, int V, SpatialBufs& B, void* stream) {
+  hipStream_t st = (hipStream_t) stream;
+  const long  o = O, v = V, vp = (V + 7) / 8 * 8;
+  hipError_t  e;
+  if((e = hipMalloc((void**) &B.xva, sizeof(double) * v * v * o * vp))) return (int) e;
   hipLaunchKernelGGL(k_antisym, dim3(nblk(v * v * o * vp)), dim3(256), 0, st, B.xva, B.xv, v, v, o,
                      vp, 0);
   if((e = hipGetLastError())) return (int) e;
   return (int) hipStreamSynchronize(st);
+}
+
+// out[a][b][i][j] = in[((a * O + i) * V + b) * O + j]: the (ai|bj) GEMM result,
+// rows (a,i) and columns (b,j), in the vvoo order.
+__global__ void k_vvoo_order(const double* __restrict__ in, double* __restrict__ out, long O, long V) {
+  const long n = V * V * O * O;
+  for(long x = (long) blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long) gridDim.x * blockDim.x) {
+    const long j = x % O, i = (x / O) % O, b = (x / (O * O)) % V, a = x / (O * O * V);
+    out[x]       = in[((a * O + i) * V + b) * O + j];
+  }
+}
+
+int launch_vvoo_order(const double* in, double* out, int O, int V, void* stream) {
+  const long n = (long) V * V * O * O;
+  hipLaunchKernelGGL(k_vvoo_order, dim3(nblk(n)), dim3(256), 0, (hipStream_t) stream, in, out,
+                     (long) O, (long) V);
+  return (int) hipGetLastError();
 }
 
 int launch_zero_u64(unsigned long long* p, long n, void* stream) {

@@ -218,6 +218,16 @@ class CCSDTriples:
                 for k in ("eo", "ev", "t1", "t2", "vvoo", "ovoo", "vvvo")]
         check(lib().xt_upload_closed_shell(self._ctx, *[_dp(a) for a in arrs]))
 
+    def upload_cholesky(self, eo, ev, t1, t2, chol):
+        """Inputs with the V2 blocks built on the GPU from Cholesky vectors
+        chol [O+V][O+V][nchol] (cholVpr order, symmetric in the orbital pair),
+        in place of V2TensorSetup::setupV2Tensors (cholesky/v2tensors.cpp:52-98)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (eo, ev, t1, t2)]
+        L = np.ascontiguousarray(chol, dtype=np.float64)
+        nmo = len(arrs[0]) + len(arrs[1])
+        assert L.ndim == 3 and L.shape[0] == nmo and L.shape[1] == nmo, L.shape
+        check(lib().xt_upload_cholesky(self._ctx, *[_dp(a) for a in arrs], L.shape[2], _dp(L)))
+
     def synth_upload(self, seed, sigma=0.05, t1_sigma=1e-3):
         check(lib().xt_synth_upload(self._ctx, seed, sigma, t1_sigma))
 

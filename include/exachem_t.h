@@ -128,6 +128,17 @@ int xt_flops(xt_ctx* ctx, double* per_task, double* total, double* parts);
  * Copied to HBM and re-laid out there for the kernels.  Must follow
  * xt_set_tiling with the same O, V.
  */
+/* Same inputs with the V2 blocks built on the device from Cholesky vectors,
+ * in place of V2TensorSetup::setupV2Tensors (cholesky/v2tensors.cpp:52-98):
+ * (pq|rs) = sum_Q chol[p][q][Q] * chol[r][s][Q] over spatial orbitals p, q,
+ * r, s in [0, O+V) (occupied first); chol is [O+V][O+V][nchol], the order of
+ * cholVpr(p, q, cind), symmetric in p, q.  eps_occ, eps_vir, t1, t2 as in
+ * xt_upload_closed_shell.  The GEMMs over the Cholesky index run on the GPU
+ * (rocBLAS); the V^3 O block never exists on the host, and the largest
+ * device temporary besides the vectors is V^2 O^2.  (O+V) * nchol and V*O
+ * must stay below 2^31 (BLAS leading dimensions). */
+int xt_upload_cholesky(xt_ctx* ctx, const double* eps_occ, const double* eps_vir,
+                       const double* t1, const double* t2, long nchol, const double* chol);
 int xt_upload_closed_shell(xt_ctx* ctx, const double* eps_occ, const double* eps_vir,
                            const double* t1, const double* t2, const double* vvoo,
                            const double* ovoo, const double* vvvo);
