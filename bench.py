@@ -117,6 +117,57 @@ def gather_leg(xt, ctx, tiling, s, n_tasks, cpu=True):
     return out
 
 
+def cholesky_leg(xt, tiling, kfac, cpu=True):
+    """V2 blocks from Cholesky vectors (v2tensors.cpp:52-98 in spatial
+    closed-shell form): seeded vectors L [O+V][O+V][K], K = kfac*(O+V),
+    symmetric in the orbital pair; xt_upload_cholesky timed end to end (host
+    to device copies of L, T1, T2, the DGEMMs over K and the operand layouts),
+    against numpy forming the same integrals on the host cores for a sample of
+    the V^3 O block, extrapolated by flops.  T2 is MP2-like from (ai|bj)."""
+    O_, V_ = tiling.O, tiling.V
+    nmo, K = O_ + V_, kfac * (O_ + V_)
+    rng = np.random.default_rng(11)
+    L = rng.standard_normal((nmo, nmo, K))
+    L += L.transpose(1, 0, 2)
+    L *= 0.5 * (0.05 / np.sqrt(K)) ** 0.5
+    eo = np.sort(rng.uniform(-2.0, -0.3, O_))
+    ev = np.sort(rng.uniform(0.1, 3.0, V_))
+    lvo = np.ascontiguousarray(L[O_:, :O_]).reshape(V_ * O_, K)
+    t0 = time.perf_counter()
+    vvoo = (lvo @ lvo.T).reshape(V_, O_, V_, O_).transpose(0, 2, 1, 3)
+    host_vvoo_s = time.perf_counter() - t0
+    D = (eo[None, None, :, None] + eo[None, None, None, :]
+         - ev[:, None, None, None] - ev[None, :, None, None])
+    t2 = np.ascontiguousarray(vvoo / D)
+    del vvoo, D
+    t1 = 1e-3 * rng.standard_normal((V_, O_))
+    ctx = xt.CCSDTriples(0)
+    ctx.set_tiling(tiling)
+    t0 = time.perf_counter()
+    ctx.upload_cholesky(eo, ev, t1, t2, L)
+    gpu_s = time.perf_counter() - t0
+    flops = 2.0 * K * (O_ * V_ * O_ * O_ + (V_ * O_) ** 2 + V_ * V_ * V_ * O_)
+    out = {"O": O_, "V": V_, "nchol": K, "chol_bytes": int(L.nbytes), "flops": flops,
+           "gpu_upload_s": gpu_s, "gpu_tflops_incl_copies": flops / gpu_s / 1e12,
+           "vvvo_bytes_avoided": int(8 * V_ ** 3 * O_)}
+    ctx.close()
+    if cpu:
+        # (ab|ci) for 8 values of a, the rows of the V^3 O block numpy forms fastest
+        lvv = L[O_:, O_:]
+        t0 = time.perf_counter()
+        for a in range(8):
+            _ = lvv[a].reshape(V_, K) @ lvo.T
+        s_cpu = time.perf_counter() - t0
+        f_cpu = 2.0 * K * 8 * V_ * V_ * O_
+        rate = f_cpu / s_cpu
+        out["cpu"] = {"value_s": flops / rate, "unit": "s (extrapolated, all blocks)",
+                      "gflops": rate / 1e9, "threads": os.cpu_count(),
+                      "sample": "(ab|ci) for 8 of the V values of a, plus (ai|bj) in "
+                                f"{host_vvoo_s:.1f}s, numpy on the host cores"}
+        out["speedup_vs_cpu"] = out["cpu"]["value_s"] / gpu_s
+    return out
+
+
 def cpu_baseline(tiling, s, per_task, budget_s):
     """Times the oracle on tasks of the workload, from the 25th flop
     percentile upwards, until the budget is spent."""
@@ -175,6 +226,9 @@ def main():
                     help="standard deviation of the synthetic T1 amplitudes")
     ap.add_argument("--gather-tasks", type=int, default=16,
                     help="tasks of the device-side gather leg (0 skips it)")
+    ap.add_argument("--cholesky-leg", type=int, default=0, metavar="KFAC",
+                    help="also time the V2 construction from KFAC*(O+V) Cholesky vectors on the "
+                         "GPU (xt_upload_cholesky) against numpy on the host cores; 0 skips it")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -315,6 +369,11 @@ def main():
         }
         log(f"fp64 {f['kernel_ms']:.1f} ms, {args.precision} {m['kernel_ms']:.1f} ms, "
             f"rel dE4 {d4 / abs(f['e4']):.2e}")
+
+    if args.cholesky_leg > 0:
+        line["cholesky_v2"] = cholesky_leg(xt, tiling, args.cholesky_leg,
+                                           cpu=not args.no_cpu_baseline)
+        log(f"cholesky v2: {line['cholesky_v2']}")
 
     if args.gather_tasks > 0:
         line["device_gather"] = gather_leg(xt, ctx, tiling, s, args.gather_tasks,
