@@ -117,6 +117,29 @@ def gather_leg(xt, ctx, tiling, s, n_tasks, cpu=True):
     return out
 
 
+def expand_leg(ctx, tiling):
+    """Dense spin-orbital T1/T2 of the workload (cd_ccsd_cs_ann.cpp:932-976)
+    written by xt_expand_amplitudes, timed with HIP events on its stream;
+    algorithmic bytes 8 written per element (the spatial amplitudes read are
+    (2*2)^2 times fewer and stay in cache)."""
+    from exachem_amd.hipmem import DevBuf, Event, Stream
+    O_, V_ = tiling.O, tiling.V
+    n1, n2 = 4 * V_ * O_, 16 * (V_ * O_) ** 2
+    b1, b2 = DevBuf(8 * n1), DevBuf(8 * n2)
+    st, e0, e1 = Stream(), Event(), Event()
+    ctx.expand_amplitudes(b1.ptr.value, b2.ptr.value, st.h)  # warm-up
+    st.sync()
+    e0.record(st.h)
+    ctx.expand_amplitudes(b1.ptr.value, b2.ptr.value, st.h)
+    e1.record(st.h)
+    ms = e1.elapsed_ms(e0)
+    for x in (b1, b2, st, e0, e1):
+        (x.free if hasattr(x, "free") else x.destroy)()
+    return {"t1_doubles": n1, "t2_doubles": n2, "gpu_ms": ms,
+            "roofline": {"bound": "hbm", "achieved": 8.0 * (n1 + n2) / ms / 1e6, "peak": 8000.0,
+                         "unit": "GB/s", "frac": 8.0 * (n1 + n2) / ms / 1e6 / 8000.0}}
+
+
 def cholesky_leg(xt, tiling, kfac, cpu=True):
     """V2 blocks from Cholesky vectors (v2tensors.cpp:52-98 in spatial
     closed-shell form): seeded vectors L [O+V][O+V][K], K = kfac*(O+V),
@@ -229,6 +252,8 @@ def main():
     ap.add_argument("--cholesky-leg", type=int, default=0, metavar="KFAC",
                     help="also time the V2 construction from KFAC*(O+V) Cholesky vectors on the "
                          "GPU (xt_upload_cholesky) against numpy on the host cores; 0 skips it")
+    ap.add_argument("--expand-leg", action="store_true",
+                    help="also time the dense spin-orbital T1/T2 expansion (xt_expand_amplitudes)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -374,6 +399,10 @@ def main():
         line["cholesky_v2"] = cholesky_leg(xt, tiling, args.cholesky_leg,
                                            cpu=not args.no_cpu_baseline)
         log(f"cholesky v2: {line['cholesky_v2']}")
+
+    if args.expand_leg:
+        line["t_expansion"] = expand_leg(ctx, tiling)
+        log(f"t expansion: {line['t_expansion']}")
 
     if args.gather_tasks > 0:
         line["device_gather"] = gather_leg(xt, ctx, tiling, s, args.gather_tasks,

@@ -98,6 +98,7 @@ def lib():
     L.xt_learn_schedule.argtypes = [vp, vp, C.POINTER(XtOpts), C.c_long, C.c_int, C.c_int,
                                     C.c_float, C.POINTER(LearnStats)]
     L.xt_upload_cholesky.argtypes = [vp, dp, dp, dp, dp, C.c_long, dp]
+    L.xt_expand_amplitudes.argtypes = [vp, vp, vp, vp]
     L.xt_gather_task.argtypes = [vp, lp, vp] + [vp] * 6 + [ip] * 7
     L.xt_packed_sizes.argtypes = [vp, lp]
     L.xt_gather_last_elems.argtypes = [vp, lp]
@@ -110,7 +111,8 @@ def lib():
                  "xt_mlp_get_params", "xt_mlp_set_params", "xt_mlp_loss_grad", "xt_mlp_train",
                  "xt_mlp_predict", "xt_task_features", "xt_set_task_cost", "xt_learn_schedule",
                  "xt_set_group_shape", "xt_get_group_shape", "xt_autotune", "xt_gather_task",
-                 "xt_packed_sizes", "xt_gather_last_elems", "xt_upload_cholesky"):
+                 "xt_packed_sizes", "xt_gather_last_elems", "xt_upload_cholesky",
+                 "xt_expand_amplitudes"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

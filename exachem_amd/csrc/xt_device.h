@@ -165,6 +165,8 @@ struct GatherDst {
 };
 int launch_gather(const GatherBlock* blocks, int n_blocks, long max_elems, const GatherSrc& S,
                   const GatherDst& D, void* stream);
+// Dense spin-orbital T1 / T2 from the spatial amplitudes (either may be null).
+int launch_expand_t(const GatherSrc& S, double* t1_so, double* t2_so, void* stream);
 
 int launch_spatial_mixed(const MixedDev& M, const RunDev& R, int grid, void* stream);
 // Scale of every contraction part of every task (sign and term coefficients

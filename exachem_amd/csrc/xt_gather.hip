@@ -7,6 +7,8 @@
 // writes every element of every block.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "xt_device.h"
 
 namespace xt {
@@ -78,7 +80,47 @@ __global__ void k_gather(const GatherBlock* __restrict__ blocks, GatherSrc S, Ga
   }
 }
 
+// Dense spin-orbital amplitudes, [alpha, beta] per space:
+//   t1_so[A][I] (2V x 2O), t2_so[A][B][I][J] ((2V)^2 (2O)^2), A = spin * V + a.
+// blockIdx.y strides over rows (A for T1, the (A, B) pair for T2), threads
+// over the occupied index (pair) of the row.
+__global__ void k_expand_t(GatherSrc S, double* __restrict__ t1_so, double* __restrict__ t2_so) {
+  const long nv = 2 * S.V, no = 2 * S.O;
+  if(t1_so)
+    for(long a = blockIdx.y; a < nv; a += gridDim.y)
+      for(long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < no; i += (long) gridDim.x * blockDim.x) {
+        const long x[4]  = {a % S.V, i % S.O, 0, 0};
+        const int  sp[4] = {(int) (a / S.V), (int) (i / S.O), 0, 0};
+        t1_so[a * no + i] = so_value(S, kGatherT1, sp, x);
+      }
+  if(t2_so) {
+    // spin = index >= n, spatial = index - spin * n: no divide per element
+    // but the 32-bit one that splits the occupied pair
+    const unsigned uo = (unsigned) S.O, uno = (unsigned) no, nq = uno * uno;
+    for(long p = blockIdx.y; p < nv * nv; p += gridDim.y) {
+      const long a = p / nv, b = p % nv;
+      const int  sa = a >= S.V, sb = b >= S.V;
+      double*    row = t2_so + p * (long) nq;
+      for(unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        const unsigned i = q / uno, j = q - i * uno;
+        const int      si = i >= uo, sj = j >= uo;
+        const long     x[4]  = {a - sa * S.V, b - sb * S.V, (long) (i - si * uo), (long) (j - sj * uo)};
+        const int      sp[4] = {sa, sb, si, sj};
+        row[q]               = so_value(S, kGatherT2, sp, x);
+      }
+    }
+  }
+}
+
 }  // namespace
+
+int launch_expand_t(const GatherSrc& S, double* t1_so, double* t2_so, void* stream) {
+  const long     no = 2 * S.O, nv = 2 * S.V;
+  const unsigned gx = (unsigned) std::min<long>((no * no + 255) / 256, 64);
+  const unsigned gy = (unsigned) std::min<long>(nv * nv, 65535);
+  hipLaunchKernelGGL(k_expand_t, dim3(gx, gy), dim3(256), 0, (hipStream_t) stream, S, t1_so, t2_so);
+  return (int) hipGetLastError();
+}
 
 int launch_gather(const GatherBlock* blocks, int n_blocks, long max_elems, const GatherSrc& S,
                   const GatherDst& D, void* stream) {

@@ -1118,6 +1118,26 @@ int xt_gather_task(xt_ctx* c, const long* task, void* stream, double* d1_t2, dou
   return XT_OK;
 }
 
+int xt_expand_amplitudes(xt_ctx* c, void* stream, double* t1_so, double* t2_so) {
+  if(!c) return fail(XT_EINVAL, "xt_expand_amplitudes: null context");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_expand_amplitudes: no tiling");
+  if(c->device < 0) return fail(XT_ENODEV, "xt_expand_amplitudes: host-only context");
+  if(!c->have_inputs) return fail(XT_ESTATE, "xt_expand_amplitudes: no inputs uploaded");
+  if(!t1_so && !t2_so) return XT_OK;
+  XT_HIP(hipSetDevice(c->device));
+  const Plan&   P = c->plan;
+  xt::GatherSrc S = {};
+  S.O             = P.O;
+  S.V             = P.V;
+  S.Op            = (P.O + 7) / 8 * 8;
+  S.Vp            = (P.V + 7) / 8 * 8;
+  S.t1            = c->in.t1;
+  S.t2h           = c->in.t2h;
+  const int err   = xt::launch_expand_t(S, t1_so, t2_so, stream);
+  if(err) return fail(XT_EHIP, std::string("xt_expand_amplitudes: ") + hipGetErrorString((hipError_t) err));
+  return XT_OK;
+}
+
 int xt_gather_last_elems(xt_ctx* c, long* n) {
   if(!c || !n) return fail(XT_EINVAL, "xt_gather_last_elems: null argument");
   *n = c->gather_elems;

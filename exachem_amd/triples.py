@@ -347,6 +347,14 @@ class CCSDTriples:
                                      "d2_exec")], _ip(counts)))
         return meta, [int(x) for x in counts]
 
+    def expand_amplitudes(self, t1_ptr, t2_ptr, stream=None):
+        """Dense spin-orbital T1 [2V][2O] and T2 [2V][2V][2O][2O] into device
+        buffers (int pointers or None), the expansion of
+        cd_ccsd_cs_ann.cpp:932-976."""
+        check(lib().xt_expand_amplitudes(self._ctx, stream,
+                                         C.c_void_p(int(t1_ptr)) if t1_ptr else None,
+                                         C.c_void_p(int(t2_ptr)) if t2_ptr else None))
+
     def gather_last_elems(self):
         """Doubles written by the last gather_task."""
         n = C.c_long()

@@ -224,6 +224,13 @@ int xt_gather_task(xt_ctx* ctx, const long task[6], void* stream, double* d1_t2,
  * size_T_d1_t2, size_T_d1_v2, size_T_d2_t2, size_T_d2_v2 (doubles) and
  * max_d1_kernels_pertask, max_d2_kernels_pertask (ccsd_t_fused_driver.hpp:176-195). */
 int xt_packed_sizes(xt_ctx* ctx, long sizes[8]);
+/* Dense spin-orbital amplitudes from the context's closed-shell ones, on
+ * `stream`: the expansion of cd_ccsd_cs_ann.cpp:932-976 (t1_aa into both spin
+ * blocks of d_t1; t2_aaaa = t2_bbbb = t_ij^ab - t_ij^ba, t2_abab = t2_baba =
+ * t_ij^ab, t2_abba = t2_baab = -t_ij^ab into d_t2), with spin-orbital index
+ * order [alpha, beta] per space.  t1_so is [2V][2O] and t2_so is
+ * [2V][2V][2O][2O], device memory; either may be NULL. */
+int xt_expand_amplitudes(xt_ctx* ctx, void* stream, double* t1_so, double* t2_so);
 /* Doubles written by the context's last xt_gather_task (the enabled blocks). */
 int xt_gather_last_elems(xt_ctx* ctx, long* n);
 

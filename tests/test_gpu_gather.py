@@ -111,3 +111,27 @@ def test_spatial_host_gather_matches_dense(xt, oracle):
         m = np.concatenate([meta[key] for key in ("s1_size", "s1_exec", "d1_size", "d1_exec",
                                                   "d2_size", "d2_exec")])
         assert np.array_equal(g.meta, m), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["uneven_small", "config0"])
+def test_amplitude_expansion_matches_oracle(xt, oracle, name):
+    """xt_expand_amplitudes (cd_ccsd_cs_ann.cpp:932-976) writes the dense
+    spin-orbital T1/T2 the oracle's expansion writes, bit for bit."""
+    from exachem_amd.hipmem import DevBuf, Stream
+    case = next(c for c in small_cases() if c["name"] == name)
+    ctx, s = load_case(xt, case)
+    so = oracle.expand_closed_shell(case["O"], case["V"], s["t1"], s["t2"], s["vvoo"], s["ovoo"],
+                                    s["vvvo"])
+    n1, n2 = so[0].size, so[1].size
+    b1, b2 = DevBuf(8 * n1), DevBuf(8 * n2)
+    st = Stream()
+    print(f"[gpu] expand {name}: t1 {so[0].shape} t2 {so[1].shape}", flush=True)
+    ctx.expand_amplitudes(b1.ptr.value, b2.ptr.value, st.h)
+    st.sync()
+    assert np.array_equal(b1.to_array(n1).view(np.uint64), so[0].ravel().view(np.uint64))
+    assert np.array_equal(b2.to_array(n2).view(np.uint64), so[1].ravel().view(np.uint64))
+    for b in (b1, b2):
+        b.free()
+    st.destroy()
+    ctx.close()
