@@ -393,30 +393,67 @@ __device__ __forceinline__ double recip(double d) {
 // s1 term of S1P slots starting at S0, staged in LDS: returns this thread's
 // sum over its elements of (t3/D) * s.  W = t3/D is already in sm.t3 (zero
 // outside the tile), T1 and V blocks are zero outside the tile.
+//
+// The thread's elements e = tid + NT*rr are every (p5, p6) = (rr & 7, rr >> 3)
+// for its own (h1, h2, h3, p4) (the bits of tid), and the slots of one sweep
+// share their particle position pa = S0/3.  So a sweep needs one T1 value and
+// 64 V values per slot (pa = p4), or 8 and 8 (pa = p5, p6): each is read from
+// LDS once instead of once per element.  Per element the slots are summed in
+// FP32 in slot order and the products W*s added in FP64 in element order, as
+// an element-by-element loop would.
 template<int S0>
 __device__ __forceinline__ double s1_sweep(const MSmem& sm) {
-  const int tid = threadIdx.x;
-  double    es  = 0.0;
-#pragma unroll 2
-  for(int rr = 0; rr < BOX / NT; rr++) {
-    const int e = tid + NT * rr;
-    int       idx[6];
+  static_assert(S0 % 3 == 0 && S1P == 3, "a sweep covers the three slots of one pa");
+  constexpr int pa = S0 / 3;
+  const int     tid = threadIdx.x;
+  const int     hidx[3] = {tid & 3, (tid >> 2) & 3, (tid >> 4) & 3};
+  const int     p4 = (tid >> 6) & 7;
+  int           vb[S1P], th[S1P];  // per slot: h part of the V index, T1 hole index
 #pragma unroll
-    for(int p = 0; p < 6; p++) idx[p] = (e >> sh(p)) & (ext(p) - 1);
-    float sv = 0.f;
+  for(int q = 0; q < S1P; q++) {
+    const int hi = (S0 + q) % 3;
+    int       hj, hk;
+    pair_of(hi, hj, hk);
+    vb[q] = hidx[hk] + 4 * hidx[hj];
+    th[q] = hidx[hi];
+  }
+  double es = 0.0;
+  if constexpr(pa == 0) {  // T1 by p4 (one value per slot); V by (pc, pb) = (p6, p5)
+    float t[S1P];
 #pragma unroll
-    for(int q = 0; q < S1P; q++) {
-      const int slot = S0 + q;
-      if(slot < 9) {
-        const int pa = slot / 3, hi = slot % 3;
-        int       hj, hk, pb, pc;
-        pair_of(hi, hj, hk);
-        pair_of(pa, pb, pc);
-        sv += sm.u.s.t1[q][idx[3 + pa] * 4 + idx[hi]] *
-              sm.u.s.v1[q][idx[hk] + 4 * idx[hj] + 16 * idx[3 + pc] + 128 * idx[3 + pb]];
+    for(int q = 0; q < S1P; q++) t[q] = sm.u.s.t1[q][p4 * 4 + th[q]];
+#pragma unroll 8
+    for(int rr = 0; rr < 64; rr++) {
+      float sv = 0.f;
+#pragma unroll
+      for(int q = 0; q < S1P; q++) sv += t[q] * sm.u.s.v1[q][vb[q] + 16 * (rr >> 3) + 128 * (rr & 7)];
+      es += (double) (sm.t3[tid + NT * rr] * sv);
+    }
+  }
+  else {
+    // pa = p5: T1 by p5, V by (pc, pb) = (p6, p4); pa = p6: T1 by p6, V by (p5, p4).
+    // The operand indexed by p5 is held for the whole sweep (constant indices
+    // in the unrolled p5 loop); the one indexed by p6 is read per p6.
+    float hold[S1P][8];
+#pragma unroll
+    for(int q = 0; q < S1P; q++)
+#pragma unroll
+      for(int k = 0; k < 8; k++)
+        hold[q][k] = pa == 1 ? sm.u.s.t1[q][k * 4 + th[q]] : sm.u.s.v1[q][vb[q] + 16 * k + 128 * p4];
+#pragma unroll 1
+    for(int p6 = 0; p6 < 8; p6++) {
+      float per[S1P];
+#pragma unroll
+      for(int q = 0; q < S1P; q++)
+        per[q] = pa == 1 ? sm.u.s.v1[q][vb[q] + 16 * p6 + 128 * p4] : sm.u.s.t1[q][p6 * 4 + th[q]];
+#pragma unroll
+      for(int p5 = 0; p5 < 8; p5++) {
+        float sv = 0.f;
+#pragma unroll
+        for(int q = 0; q < S1P; q++) sv += pa == 1 ? hold[q][p5] * per[q] : per[q] * hold[q][p5];
+        es += (double) (sm.t3[tid + NT * (p5 + 8 * p6)] * sv);
       }
     }
-    es += (double) (sm.t3[e] * sv);
   }
   return es;
 }
