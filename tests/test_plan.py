@@ -173,3 +173,24 @@ def test_group_shape_validation(xt):
     with pytest.raises(xt.XtError):
         ctx.set_group_shape(5, [1, 1, 1, 1, 1, 1])
     ctx.close()
+
+
+def test_gather_and_expansion_errors(xt):
+    """Host-only contexts and bad task tuples are errors, not crashes, for
+    the device gather, the amplitude expansion and the Cholesky upload."""
+    L = xt.lib()
+    ctx = xt.CCSDTriples(-1)
+    ctx.set_tiling(xt.Tiling(8, 24, [4, 4], [12, 12]))
+    one = C.c_void_p(8)
+    z = np.zeros(9 * 16, dtype=np.int32)
+    zp = z.ctypes.data_as(C.POINTER(C.c_int))
+    task = np.array([0, 0, 0, 4, 4, 4], dtype=np.int64)
+    tp = task.ctypes.data_as(C.POINTER(C.c_long))
+    rc = L.xt_gather_task(ctx._ctx, tp, None, *[one] * 6, *[zp] * 7)
+    assert rc == xt.XT_ENODEV
+    rc = L.xt_gather_task(ctx._ctx, tp, None, *[None] * 6, *[zp] * 7)
+    assert rc == xt.XT_EINVAL
+    assert L.xt_expand_amplitudes(ctx._ctx, None, one, one) == xt.XT_ENODEV
+    sz = ctx.packed_sizes()
+    assert sz["max_d1"] == 9 * 2 and sz["max_d2"] == 9 * 2  # alpha tiles per space
+    ctx.close()
