@@ -1,7 +1,7 @@
-"""Writes the golden fixtures of the GPU parity tests (run here, on the CPU).
+"""Writes the golden fixtures of the GPU parity tests (runs on the CPU).
 
 Inputs come from the product's seeded generator (xt_synth_closed_shell), so
-the GPU tests regenerate the identical arrays on the GPU box; the fixture
+the GPU tests regenerate the identical arrays on the GPU host; the fixture
 stores checksums of every input array to prove that.  Expected energies come
 from the CPU oracle (oracle/ccsd_t_oracle.cpp, the restatement of ExaChem's
 CPU fused (T) path).
