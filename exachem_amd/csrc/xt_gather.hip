@@ -55,28 +55,101 @@ __device__ __forceinline__ double so_value(const GatherSrc& S, int kind, const i
   }
 }
 
-// blockIdx.y = block; threads stride over its elements in destination order.
-// Destination element e has (row-major, last fastest) coordinates i[k] over
-// the permuted extents sizeA[perm[k]]; source coordinate perm[k] is i[k]
-// (hptt's row-major permutation), and the value is alpha times the element.
-__global__ void k_gather(const GatherBlock* __restrict__ blocks, GatherSrc S, GatherDst D) {
-  const GatherBlock& B = blocks[blockIdx.y];
+// Source dimension whose consecutive values are adjacent in memory, for the
+// term the block reads (for a T2 block that reads both terms, the first one).
+__device__ __forceinline__ int fast_dim(const GatherBlock& B) {
+  switch(B.kind) {
+    case kGatherIJKA: return 2;
+    case kGatherT2: return (B.spin[0] == B.spin[2] && B.spin[1] == B.spin[3]) ? 3 : 2;
+    default: return 1;  // T1, IJAB, IABC
+  }
+}
+
+constexpr int GT   = 256;      // threads per gather workgroup
+constexpr int SLAB = 48 * 49;  // LDS slab (doubles): extents up to 48 with one pad column
+
+// blockIdx.y = block.  Destination element e has (row-major, last fastest)
+// coordinates i[k] over the permuted extents size[perm[k]]; source coordinate
+// perm[k] is i[k] (hptt's row-major permutation), and the value is alpha
+// times the element.
+//
+// Where the destination's fastest dimension is also the source's contiguous
+// one, threads stride over destination elements directly.  Otherwise the block
+// goes through LDS in slabs over (destination position of the source's
+// contiguous dimension) x (destination's fastest position), one slab per pair
+// of the two other positions: the slab is read along the source's contiguous
+// dimension and written along the destination's, so both sides run through
+// whole rows instead of one element per row.
+__global__ __launch_bounds__(GT) void k_gather(const GatherBlock* __restrict__ blocks, GatherSrc S,
+                                               GatherDst D) {
+  __shared__ double  slab[SLAB];
+  const GatherBlock& B   = blocks[blockIdx.y];
   double*            dst = D.buf[B.buf] + B.dst;
-  const int          nd = B.nd;
-  const unsigned     n  = (unsigned) B.n;  // a block holds fewer than 2^31 doubles (host check)
-  // 32-bit index decode: a 64-bit divide costs several times more
-  for(unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-    unsigned r = e;
-    long     x[4] = {0, 0, 0, 0};
-    int      sp[4] = {0, 0, 0, 0};
-    for(int k = nd - 1; k >= 0; k--) {
-      const int      d  = B.perm[k];
-      const unsigned sz = (unsigned) B.size[d];
-      x[d]              = B.start[d] + (long) (r % sz);
-      sp[d]             = B.spin[d];
-      r /= sz;
+  const int          nd  = B.nd;
+  const unsigned     n   = (unsigned) B.n;  // a block holds fewer than 2^31 doubles (host check)
+  int                sp[4] = {0, 0, 0, 0};
+#pragma unroll
+  for(int d = 0; d < 4; d++) sp[d] = d < nd ? B.spin[d] : 0;
+  // destination positions padded to four (leading extents 1): source
+  // dimension src[k] (-1 for padding) and extent E[k]
+  int src[4], E[4];
+#pragma unroll
+  for(int k = 0; k < 4; k++) {
+    const int kk = k - (4 - nd);
+    src[k]       = kk >= 0 ? B.perm[kk] : -1;
+    E[k]         = kk >= 0 ? B.size[src[k]] : 1;
+  }
+  const int fd = fast_dim(B);
+  int       kS = 3;
+#pragma unroll
+  for(int k = 0; k < 3; k++)
+    if(src[k] == fd) kS = k;
+  const unsigned ES = (unsigned) E[kS], EF = (unsigned) E[3], ld = EF + 1;
+  if(kS == 3 || ES * ld > (unsigned) SLAB) {
+    // 32-bit index decode: a 64-bit divide costs several times more
+    for(unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+      unsigned r = e;
+      long     x[4] = {0, 0, 0, 0};
+      for(int k = nd - 1; k >= 0; k--) {
+        const int      d  = B.perm[k];
+        const unsigned sz = (unsigned) B.size[d];
+        x[d]              = B.start[d] + (long) (r % sz);
+        r /= sz;
+      }
+      dst[e] = B.alpha * so_value(S, B.kind, sp, x);
     }
-    dst[e] = B.alpha * so_value(S, B.kind, sp, x);
+    return;
+  }
+  // the two other positions o1 < o2 index the slabs
+  const int o1 = kS == 0 ? 1 : 0, o2 = kS == 2 ? 1 : 2;
+  unsigned  st[4];
+  st[3] = 1;
+#pragma unroll
+  for(int k = 2; k >= 0; k--) st[k] = st[k + 1] * (unsigned) E[k + 1];
+  const unsigned E2 = (unsigned) E[o2], nslab = (unsigned) E[o1] * E2, nel = ES * EF;
+  const int      d1 = src[o1], d2 = src[o2], dF = src[3];
+  long           x0[4];
+#pragma unroll
+  for(int d = 0; d < 4; d++) x0[d] = d < nd ? B.start[d] : 0;
+  // every thread of the workgroup runs the same slab loop, so the barriers
+  // below are reached by all of them
+  for(unsigned sl = blockIdx.x; sl < nslab; sl += gridDim.x) {
+    const unsigned a = sl / E2, b = sl - a * E2;
+    __syncthreads();  // the previous slab's LDS reads are done
+    for(unsigned e = threadIdx.x; e < nel; e += GT) {
+      const unsigned f = e / ES, s = e - f * ES;
+      long           x[4];
+#pragma unroll
+      for(int d = 0; d < 4; d++)
+        x[d] = x0[d] + (d == fd ? s : d == dF ? f : d == d1 ? a : d == d2 ? b : 0u);
+      slab[s * ld + f] = B.alpha * so_value(S, B.kind, sp, x);
+    }
+    __syncthreads();
+    const unsigned base = a * st[o1] + b * st[o2];
+    for(unsigned e = threadIdx.x; e < nel; e += GT) {
+      const unsigned s = e / EF, f = e - s * EF;
+      dst[base + s * st[kS] + f] = slab[s * ld + f];
+    }
   }
 }
 
@@ -128,7 +201,7 @@ int launch_gather(const GatherBlock* blocks, int n_blocks, long max_elems, const
   if(n_blocks > 65535) return (int) hipErrorInvalidConfiguration;
   const long     per = (max_elems + 255) / 256;
   const unsigned gx  = (unsigned) (per < 512 ? (per > 0 ? per : 1) : 512);
-  hipLaunchKernelGGL(k_gather, dim3(gx, (unsigned) n_blocks), dim3(256), 0, (hipStream_t) stream,
+  hipLaunchKernelGGL(k_gather, dim3(gx, (unsigned) n_blocks), dim3(GT), 0, (hipStream_t) stream,
                      blocks, S, D);
   return (int) hipGetLastError();
 }
