@@ -7,6 +7,7 @@ from the CPU oracle (oracle/ccsd_t_oracle.cpp, the restatement of ExaChem's
 CPU fused (T) path).
 
     python tests/golden/make_golden.py small      # config 0 + small uneven cases (~1 min)
+    python tests/golden/make_golden.py wide       # adds the wide-tile case to the same file
     python tests/golden/make_golden.py c1sample   # sampled tasks of O=40,V=300 (long)
 """
 import json
@@ -67,6 +68,18 @@ def small():
         json.dump(cases, f, indent=1)
 
 
+def wide():
+    """A one-orbital occupied tile next to a 6-wide one, and a 50-wide virtual
+    tile (wider than one 48-element gather slab, seven 8-wide boxes of the
+    mixed kernel) next to a 3-wide one.  Added to parity_small.json without
+    recomputing the other cases."""
+    path = os.path.join(HERE, "parity_small.json")
+    cases = [c for c in json.load(open(path)) if c["name"] != "wide_tiles"]
+    cases.append(oracle_case("wide_tiles", 7, 53, [6, 1], [50, 3], SEED + 3))
+    with open(path, "w") as f:
+        json.dump(cases, f, indent=1)
+
+
 def c1sample():
     occ, vir = X.triples.uniform_tiles(40, 16), X.triples.uniform_tiles(300, 32)
     t = O.Tiling(occ, occ, vir, vir)
@@ -99,4 +112,4 @@ def c1sample():
 
 
 if __name__ == "__main__":
-    {"small": small, "c1sample": c1sample}[sys.argv[1]]()
+    {"small": small, "wide": wide, "c1sample": c1sample}[sys.argv[1]]()

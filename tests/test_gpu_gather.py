@@ -39,7 +39,8 @@ def test_packed_sizes_match_driver(xt, oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["uneven_small", "uneven_small_unrestricted", "config0"])
+@pytest.mark.parametrize("name", ["uneven_small", "uneven_small_unrestricted", "config0",
+                                  "wide_tiles"])
 def test_device_gather_matches_host_gather(xt, oracle, name):
     from exachem_amd.hipmem import DevBuf, Stream
     case = next(c for c in small_cases() if c["name"] == name)

@@ -113,6 +113,33 @@ def test_special_cases(oracle):
     assert oracle.run_spatial(t, _evl(s), zero)[:2] == (0.0, 0.0)
 
 
+def test_pauli_forbidden_tasks_vanish(oracle):
+    """Two holes (or two particles) in the same one-orbital tile: t3 vanishes
+    identically, so the task's E[4] and E[5] are 0 up to FP64 rounding of the
+    cancelling terms, squared (exactly 0 for the holes in this evaluation
+    order, ~1e-36 of the largest task for the particles).  Each such element
+    is a sum of d1/d2/s1 terms that cancel in pairs exchanging the two
+    indices, so a wrong sign or slot on one equation leaves it at the scale
+    of an ordinary task; every other task is at least 1e-6 of the largest."""
+    s = R.synth_spatial(4, 9, 5)
+    occ, vir = [3, 1], [7, 1, 1]
+    t = oracle.Tiling(occ, occ, vir, vir)
+    tasks, _ = oracle.task_list(t)
+    _, _, pt = oracle.run_spatial(t, _evl(s), s)
+    tiles = np.asarray(t.tiles)
+
+    def repeated_single(ix):
+        return any(ix[i] == ix[i + 1] and tiles[ix[i]] == 1 for i in range(2))
+
+    forbidden = np.array([repeated_single(k[:3]) or repeated_single(k[3:]) for k in tasks])
+    assert forbidden.sum() >= 4 and (~forbidden).sum() >= 4
+    big = np.max(np.abs(pt))
+    assert np.all(np.abs(pt[forbidden]) <= 1e-28 * big)
+    assert np.all(np.abs(pt[~forbidden]) >= 1e-6 * big)
+    holes = np.array([k[0] == k[1] == 1 or k[1] == k[2] == 1 for k in tasks])
+    assert holes.sum() >= 4 and np.all(pt[holes] == 0.0)
+
+
 def test_mp4_triples_identity(oracle):
     """With first-order T2 (t = (ia|jb)/D) and T1 = 0, E[4] is the MP4 triples
     energy; checked against the spatial formula with T1 = 0 (no shared code)."""
