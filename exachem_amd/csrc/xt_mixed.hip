@@ -366,11 +366,28 @@ __device__ __forceinline__ void box_contract(MSmem& sm, const MixedDev& M, uint6
       mfma_chunk(ring[s], F[s & 1], acc);
       stage(sm, ring[(s + 2) % PF], s & 1);
       if(++cch == cn) {
-        if(cmask) flush(sm, cgp >> 1, ring[s].sc, acc);
-        cch = 0;
+        int ngp = cgp;
         do {
-          ++cgp;
-        } while(cgp < 18 && !part_on(d1, d2, cgp));
+          ++ngp;
+        } while(ngp < 18 && !part_on(d1, d2, ngp));
+        // A d1 part followed by the d2 part of the same bipartition adds into
+        // the same box elements: its accumulators carry over, rescaled by the
+        // ratio of the two part scales (both +-2^k, so the ratio is exact;
+        // the next chunk, first of the d2 part, holds the d2 scale).  Ratios
+        // outside 2^+-60 flush instead, keeping the FP32 range.
+        bool carry = false;
+        if(!(cgp & 1) && ngp == cgp + 1) {
+          const float r = ring[s].sc / ring[(s + 1) % PF].sc;
+          carry         = fabsf(r) >= 0x1p-60f && fabsf(r) <= 0x1p60f;
+          if(carry)
+#pragma unroll
+            for(int j = 0; j < 4; j++)
+#pragma unroll
+              for(int q = 0; q < 16; q++) acc[j][q] *= r;
+        }
+        if(!carry && cmask) flush(sm, cgp >> 1, ring[s].sc, acc);
+        cch = 0;
+        cgp = ngp;
         cn    = cgp < 18 ? ((cgp & 1) ? M.Kv / KC : M.Ko / KC) : 0x7fffffff;
         cmask = __builtin_amdgcn_readfirstlane(cgp < 18 ? tile_mask(sm, cgp >> 1) : 0u);
       }
