@@ -23,10 +23,14 @@
 //     bipartition boundaries; every thread issues the same loads for every
 //     chunk (rows outside the tile read a zero row), so the compiler's wait
 //     counts follow the ring; one barrier per K chunk;
-//   * MFMA tiles wholly outside the task tile (row tile of an out-of-range
-//     pB, column tile of out-of-range ps) are skipped;
+//   * MFMAs are not skipped: rows and columns outside the task tile read the
+//     zero row and add zeros (a branch per MFMA costs more issue time than
+//     the skipped work; a branch per chunk for a wave whose row tile is
+//     outside makes the compiler spill 25 registers);
 //   * the flush into the box adds precomputed per-lane, per-row and
-//     per-column index parts (disjoint bit fields of the box index);
+//     per-column index parts (disjoint bit fields of the box index); the d1
+//     part of a bipartition carries its accumulators into the d2 part, so a
+//     bipartition flushes once;
 //   * energy: one FP64 reciprocal of the denominator per element (from
 //     separable hole and particle sums), E4 in FP64, then t3/D is kept in
 //     place for the s1 sweeps (FP32 products, FP64 sums); task partials are
