@@ -22,6 +22,7 @@ kernel's HIP-event time.  Beside it:
     oracle, task by task).
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -115,6 +116,65 @@ def gather_leg(xt, ctx, tiling, s, n_tasks, cpu=True):
         out["cpu_doubles_written"] = int(n_cpu)
         out["speedup_vs_cpu"] = cpu_ms / gpu_ms
     return out
+
+
+def dropin_leg(xt, ctx, tiling, s, per_task, n_tasks):
+    """The per-task drop-in path end to end on the device for a seeded sample
+    of the workload's tasks: xt_gather_task writes the packed buffers
+    (ccsd_t_data_{s1,d1,d2}_new) and xt_fully_fused_ccsd_t_gpu runs the
+    reference launcher's contract on them (ccsd_t_all_fused.hpp:190-267),
+    each timed with HIP events on their stream.  Beside it, the persistent
+    FP64 kernel (xt_run_tasks) on the same tasks.  Each task's energy is the
+    factor times the sum of its block partials, as hostEnergyReduce does."""
+    from exachem_amd.hipmem import DevBuf, Event, Stream
+    tasks, fac = ctx.task_list()
+    rng = np.random.default_rng(7)
+    pick = np.sort(rng.choice(len(tasks), min(n_tasks, len(tasks)), replace=False))
+    sz = ctx.packed_sizes()
+    names = ("d1_t2", "d1_v2", "d2_t2", "d2_v2", "s1_t1", "s1_v2")
+    dev = {n: DevBuf(8 * sz[n]) for n in names}
+    ptrs = {n: dev[n].ptr.value for n in names}
+    max_dims = {n: sz[n] // (9 if n.startswith("s1") else
+                             (sz["max_d1"] if n.startswith("d1") else sz["max_d2"]))
+                for n in names}
+    tiles = list(tiling.occ) * 2 + list(tiling.vir) * 2
+    offs = np.concatenate([[0], np.cumsum(tiles)]).astype(np.int64)
+    evl = DevBuf.from_array(np.concatenate([s["eo"], s["eo"], s["ev"], s["ev"]]))
+    noab, nvab = 2 * len(tiling.occ), 2 * len(tiling.vir)
+    st, e0, e1, e2 = Stream(), Event(), Event(), Event()
+    gather_ms, kernel_ms, e = 0.0, 0.0, np.zeros((len(pick), 2))
+    for i, k in enumerate([pick[0]] + list(pick)):  # first pass: warm-up
+        dims = [int(tiles[b]) for b in tasks[k]]
+        nb = int(np.prod([(d + 3) // 4 for d in dims]))
+        part = DevBuf(16 * nb)
+        ev = [C.c_void_p(evl.ptr.value + 8 * int(offs[b])) for b in tasks[k]]
+        e0.record(st.h)
+        m, _ = ctx.gather_task(tasks[k], ptrs, st.h)
+        e1.record(st.h)
+        xt.fully_fused_ccsd_t_gpu(st.h, nb, dims, [dev[n].ptr for n in names], m, noab, nvab,
+                                  max_dims, ev, part.ptr)
+        e2.record(st.h)
+        p = part.to_array(2 * nb)  # synchronous: the stream's work is done
+        part.free()
+        if i > 0:
+            gather_ms += e1.elapsed_ms(e0)
+            kernel_ms += e2.elapsed_ms(e1)
+            e[i - 1] = fac[k] * np.sum(p[:nb]), fac[k] * np.sum(p[nb:])
+    for b in list(dev.values()) + [evl]:
+        b.free()
+    for x in (st, e0, e1, e2):
+        x.destroy()
+    r = ctx.run_tasks(pick, precision=xt.PREC_FP64)
+    flops = float(per_task[pick].sum())
+    d = np.abs(e - r["task_partials"]) / np.maximum(np.abs(r["task_partials"][:, :1]), 1.0)
+    tf = flops / kernel_ms / 1e9
+    return {"tasks": int(len(pick)), "flops": flops, "gather_ms": gather_ms,
+            "packed_kernel_ms": kernel_ms, "packed_tflops": tf,
+            "persistent_fp64_ms": r["kernel_ms"], "persistent_fp64_tflops":
+            flops / r["kernel_ms"] / 1e9,
+            "max_rel_dE_task_vs_persistent": float(d.max()),
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": 78.6, "unit": "TFLOP/s",
+                         "frac": tf / 78.6, "kernel": "k_packed_fp64"}}
 
 
 def expand_leg(ctx, tiling):
@@ -249,6 +309,9 @@ def main():
                     help="standard deviation of the synthetic T1 amplitudes")
     ap.add_argument("--gather-tasks", type=int, default=16,
                     help="tasks of the device-side gather leg (0 skips it)")
+    ap.add_argument("--dropin-tasks", type=int, default=8,
+                    help="tasks of the per-task drop-in leg: device gather plus "
+                         "xt_fully_fused_ccsd_t_gpu (0 skips it)")
     ap.add_argument("--cholesky-leg", type=int, default=0, metavar="KFAC",
                     help="also time the V2 construction from KFAC*(O+V) Cholesky vectors on the "
                          "GPU (xt_upload_cholesky) against numpy on the host cores; 0 skips it")
@@ -408,6 +471,10 @@ def main():
         line["device_gather"] = gather_leg(xt, ctx, tiling, s, args.gather_tasks,
                                            cpu=not args.no_cpu_baseline)
         log(f"device gather: {line['device_gather']}")
+
+    if args.dropin_tasks > 0:
+        line["dropin_per_task"] = dropin_leg(xt, ctx, tiling, s, per_task, args.dropin_tasks)
+        log(f"per-task drop-in: {line['dropin_per_task']}")
 
     if not args.no_cpu_baseline:
         cb = cpu_baseline(tiling, s, per_task, args.cpu_budget)

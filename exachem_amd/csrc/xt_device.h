@@ -195,6 +195,9 @@ int mlp_forward(const float* P, const float* X, long n, float* Y, float* A1, flo
 int mlp_loss_grad(const MlpWork& W, long n, void* stream);
 int mlp_train_step(const MlpWork& W, long n, float lr, float* loss_hist, void* stream);
 
+// Tile-count limits of the per-task launcher (the reference's, nontc:56-57);
+// the host rejects larger noab / nvab before any launch.
+constexpr int kMaxNoab = 50, kMaxNvab = 140;
 struct PackedArgs {
   int           dim[6];
   const double* d1_t2;

@@ -1345,7 +1345,8 @@ int xt_learn_schedule(xt_ctx* c, xt_mlp* m, const xt_opts* opts, long n_sample, 
 
 // ------------------------------------------------------ per-task drop-in
 namespace {
-constexpr int kMaxNoab = 50, kMaxNvab = 140;  // reference limits (nontc:56-57)
+using xt::kMaxNoab;
+using xt::kMaxNvab;
 constexpr int kMetaInts = 16 * kMaxNoab + 16 * kMaxNvab + 15;
 constexpr int kRing     = 4;
 

@@ -639,6 +639,8 @@ int launch_reduce(const TaskDev* tasks, int n_tasks, const unsigned long long* a
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
   __shared__ Smem sm;
+  __shared__ int4 klist[kMaxNoab + kMaxNvab + 1];  // tiles of one bipartition's K, + sentinel
+  __shared__ int  s_nk;
   const int       tid = threadIdx.x, row = tid >> 2, kq = tid & 3;
   const int*      dim = A.dim;
   int             nsub[6], sub[6];
@@ -689,48 +691,84 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
     for(int i = 0; i < 2; i++)
 #pragma unroll
       for(int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-    bool any = false;
-    // d1 tiles (h7b), then d2 tiles (p7b)
-#pragma unroll 1
-    for(int fam = 0; fam < 2; fam++) {
-      const int ntile = fam == 0 ? A.noab : A.nvab;
-#pragma unroll 1
-      for(int tb = 0; tb < ntile; tb++) {
-        const int ex = fam == 0 ? d1_exec[s1d + 9 * tb] : d2_exec[s2d + 9 * tb];
-        if(ex < 0) continue;
-        any          = true;
-        const int K  = fam == 0 ? d1_size[3 + 7 * tb] : d2_size[6 + 7 * tb];
-        const double* abase;
-        const double* bbase;
-        long          astr = 1, bstr = 1;  // stride of k
-        if(fam == 0) {
-          abase = A.d1_t2 + A.max_d1_t2 * (long) ex + (gPA + (gPB + gHS * DPB) * DPA) * (long) K;
-          bbase = A.d1_v2 + A.max_d1_v2 * (long) ex + (gHB + (gHA + gPS * DHA) * DHB);
-          bstr  = DPS * DHA * DHB;
+    // K of this bipartition: the enabled d1 tiles (h7b), then the enabled d2
+    // tiles (p7b), packed end to end and cut into chunks of KC, so that only
+    // the last chunk is padded.  Thread 0 lists the tiles (family, exec slot,
+    // K, first packed k) with a sentinel holding the total; each thread walks
+    // the list with its own tile index, which only moves forward since its k
+    // only grow.  The next chunk is loaded into registers while the current
+    // one is multiplied.  Every thread has read the previous bipartition's
+    // list and count before thread 0 rewrites them.
+    __syncthreads();
+    if(tid == 0) {
+      int n = 0, kb = 0;
+      for(int tb = 0; tb < A.noab; tb++) {
+        const int ex = d1_exec[s1d + 9 * tb];
+        if(ex >= 0) {
+          klist[n++] = make_int4(0, ex, d1_size[3 + 7 * tb], kb);
+          kb += d1_size[3 + 7 * tb];
         }
-        else {
-          abase = A.d2_v2 + A.max_d2_v2 * (long) ex + (gHS + (gPB + gPA * DPB) * DHS) * (long) K;
-          bbase = A.d2_t2 + A.max_d2_t2 * (long) ex + (gPS + (gHA + gHB * DHA) * DPS) * (long) K;
+      }
+      for(int tb = 0; tb < A.nvab; tb++) {
+        const int ex = d2_exec[s2d + 9 * tb];
+        if(ex >= 0) {
+          klist[n++] = make_int4(1, ex, d2_size[6 + 7 * tb], kb);
+          kb += d2_size[6 + 7 * tb];
         }
-        const int nc = cdiv(K, KC);
-#pragma unroll 1
-        for(int c = 0; c < nc; c++) {
-          double av[8], bv[8];
+      }
+      klist[n] = make_int4(0, 0, 0, kb);
+      s_nk     = n;
+    }
+    __syncthreads();
+    const int  nk = s_nk, kall = klist[nk].w;
+    const bool any = nk > 0;
+    // operand rows of this thread: d1_t2 and d2_v2 / d2_t2 rows are times the
+    // tile's K; the d1_v2 element of k sits at rB1 + k * bstr1
+    const long rA1 = gPA + (gPB + gHS * DPB) * DPA, rB1 = gHB + (gHA + gPS * DHA) * DHB;
+    const long bstr1 = DPS * DHA * DHB;
+    const long rA2 = gHS + (gPB + gPA * DPB) * DHS, rB2 = gPS + (gHA + gHB * DHA) * DPS;
+    int        ti = 0;
+    int4       te = klist[0];
+    int        nbeg = klist[1].w;
+    double     av[8], bv[8];
+    auto       load = [&](int c) {
 #pragma unroll
-          for(int j = 0; j < 8; j++) {
-            const int k = c * KC + 8 * kq + j;
-            av[j]       = (mok && k < K) ? sg * abase[k * astr] : 0.0;
-            bv[j]       = (nok && k < K) ? bbase[k * bstr] : 0.0;
+      for(int j = 0; j < 8; j++) {
+        const int k = c * KC + 8 * kq + j;
+        double    a = 0.0, b = 0.0;
+        if(k < kall) {
+          while(k >= nbeg) {
+            te   = klist[++ti];
+            nbeg = klist[ti + 1].w;
           }
-          __syncthreads();
-#pragma unroll
-          for(int j = 0; j < 8; j++) {
-            As[row * LDK + 8 * kq + j] = av[j];
-            Bs[row * LDK + 8 * kq + j] = bv[j];
+          const long kl = k - te.w, K = te.z;
+          if(te.x == 0) {
+            if(mok) a = sg * A.d1_t2[A.max_d1_t2 * (long) te.y + rA1 * K + kl];
+            if(nok) b = A.d1_v2[A.max_d1_v2 * (long) te.y + rB1 + kl * bstr1];
           }
-          __syncthreads();
-          mfma_chunk(As, Bs, acc);
+          else {
+            if(mok) a = sg * A.d2_v2[A.max_d2_v2 * (long) te.y + rA2 * K + kl];
+            if(nok) b = A.d2_t2[A.max_d2_t2 * (long) te.y + rB2 * K + kl];
+          }
         }
+        av[j] = a;
+        bv[j] = b;
+      }
+    };
+    if(any) {
+      const int nc = cdiv(kall, KC);
+      load(0);
+#pragma unroll 1
+      for(int c = 0; c < nc; c++) {
+        __syncthreads();
+#pragma unroll
+        for(int j = 0; j < 8; j++) {
+          As[row * LDK + 8 * kq + j] = av[j];
+          Bs[row * LDK + 8 * kq + j] = bv[j];
+        }
+        __syncthreads();
+        if(c + 1 < nc) load(c + 1);
+        mfma_chunk(As, Bs, acc);
       }
     }
     if(any) acc_to_t3(sm, g, acc);
