@@ -94,6 +94,9 @@ struct MixedDev {
 // XCD share in its L2.  A task has 64 * prod(ceil(nsub[p] / 2^glog[p]))
 // tickets; tickets outside the task's box grid are skipped.
 constexpr int kQueues = 8;
+// Threads of the one workgroup that sums the task energies in a fixed order
+// (k_reduce_final); xt_sum_task_energies repeats the same order on the host.
+constexpr int kReduceThreads = 256;
 constexpr int kGroupLog2 = 6;  // 64 boxes per ticket group
 
 struct RunDev {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -993,6 +994,35 @@ int xt_run(xt_ctx* c, const xt_opts* opts, double* e45, double* task_partials, f
   XT_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
   if(o.record_task_time && task_ms) return task_times(c, nt, task_ms);
+  return XT_OK;
+}
+
+int xt_sum_task_energies(long n, const double* task_partials, double* e45) {
+  if(n < 0 || !e45 || (n > 0 && !task_partials))
+    return fail(XT_EINVAL, "xt_sum_task_energies: null argument or negative count");
+  if(n > INT_MAX) return fail(XT_EINVAL, "xt_sum_task_energies: more than INT_MAX tasks");
+  // the order of k_reduce_final: kReduceThreads runs of ceil(n / kReduceThreads)
+  // consecutive tasks, each summed in task order, then a halving tree
+  constexpr int kT = xt::kReduceThreads;
+  double        ra[kT], rb[kT];
+  const long    per = (n + kT - 1) / kT;
+  for(int i = 0; i < kT; i++) {
+    double     a = 0.0, b = 0.0;
+    const long lo = i * per, hi = std::min(n, lo + per);
+    for(long t = lo; t < hi; t++) {
+      a += task_partials[2 * t];
+      b += task_partials[2 * t + 1];
+    }
+    ra[i] = a;
+    rb[i] = b;
+  }
+  for(int s = kT / 2; s >= 1; s >>= 1)
+    for(int i = 0; i < s; i++) {
+      ra[i] += ra[i + s];
+      rb[i] += rb[i + s];
+    }
+  e45[0] = ra[0];
+  e45[1] = rb[0];
   return XT_OK;
 }
 

@@ -447,7 +447,10 @@ __global__ void k_zero_u64(unsigned long long* p, long n) {
   if(i < n) p[i] = 0ull;
 }
 
-// Fixed-shape sum over tasks in task order (one workgroup of NT threads).
+// Fixed-shape sum over tasks in task order (one workgroup of NT threads):
+// thread i sums tasks [i*per, (i+1)*per) in order, per = ceil(n/NT), then a
+// halving tree over the threads.
+static_assert(NT == kReduceThreads, "the host repeats this reduction with kReduceThreads");
 __global__ void __launch_bounds__(NT) k_reduce_final(const double* task_e, int n, double* e45) {
   __shared__ double ra[NT], rb[NT];
   const int         per = (n + NT - 1) / NT;

@@ -159,6 +159,17 @@ def synth_closed_shell(O, V, seed, sigma=0.05, t1_sigma=1e-3):
     return out
 
 
+def sum_task_energies(task_partials):
+    """(E4, E5) from per-task contributions (n x 2, task-list order, factor
+    included) in the fixed order of xt_run's device reduction: the partials
+    of a run completed piecewise (xt_run_tasks over the remaining tasks) give
+    the bit-identical totals of one full run (xt_sum_task_energies)."""
+    p = np.ascontiguousarray(task_partials, dtype=np.float64).reshape(-1, 2)
+    e = np.zeros(2)
+    check(lib().xt_sum_task_energies(len(p), _dp(p), _dp(e)))
+    return float(e[0]), float(e[1])
+
+
 def default_opts(**kw):
     o = XtOpts()
     lib().xt_default_opts(C.byref(o))

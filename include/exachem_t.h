@@ -189,6 +189,17 @@ int xt_run(xt_ctx* ctx, const xt_opts* opts, double* e45, double* task_partials,
 int xt_run_tasks(xt_ctx* ctx, const xt_opts* opts, long n, const long* which, double* e45,
                  double* task_partials, float* task_ms);
 
+/* E[4], E[5] from per-task contributions: task_partials[2*n] in task-list
+ * order, factor included, as xt_run returns them.  The sum follows exactly
+ * the fixed order of xt_run's device reduction.  A task's contribution is an
+ * exact fixed-point sum and does not depend on which other tasks share its
+ * run.  So a run that stopped part-way, with its finished tasks' partials
+ * kept and the rest computed by xt_run_tasks, gives the bit-identical totals
+ * of one full xt_run.  Host only; n = 0 gives 0.  No counterpart in the
+ * reference, whose (T) restart re-reads T1/T2/V2 from disk and recomputes
+ * every task (ccsd_t.cpp:162-201). */
+int xt_sum_task_energies(long n, const double* task_partials, double* e45);
+
 /* Slot codes of one task (introspection for tests and tools): codes[0] d1,
  * codes[1] d2 (6 bits per output bipartition g = 3*hs+ps: enabled, negative,
  * a1, a2, b1, b2), codes[2] s1 (4 bits per slot 3*pa+hi: enabled, negative,

@@ -283,3 +283,26 @@ def test_mixed_absolute_error_at_physical_scale(xt):
     assert abs(m["e4"] - f["e4"]) <= 1e-6
     assert abs(m["e5"] - f["e5"]) <= 1e-6
     ctx.close()
+
+
+def test_resume_from_partials_is_bit_identical(xt):
+    """A run that stops part-way resumes from its finished tasks' partials:
+    xt_run_tasks over the rest, then xt_sum_task_energies, gives the
+    bit-identical per-task partials and totals of one full xt_run (both
+    precisions).  The per-task fixed-point sums do not depend on which tasks
+    share a run, and the host sum repeats the device reduction's order."""
+    case = next(c for c in small_cases() if c["name"] == "uneven_small")
+    ctx, _ = load_case(xt, case)
+    n = ctx.num_tasks()
+    done = np.arange(0, n, 3)
+    rest = np.setdiff1d(np.arange(n), done)
+    for prec in (xt.PREC_FP64, xt.PREC_MIXED):
+        print(f"[gpu] resume from partials, precision {prec}, {len(done)} + {len(rest)} tasks",
+              flush=True)
+        full = ctx.run(precision=prec)
+        parts = np.zeros((n, 2))
+        parts[done] = ctx.run_tasks(done, precision=prec)["task_partials"]
+        parts[rest] = ctx.run_tasks(rest, precision=prec)["task_partials"]
+        assert np.array_equal(parts.view(np.uint64), full["task_partials"].view(np.uint64))
+        assert xt.sum_task_energies(parts) == (full["e4"], full["e5"])
+    ctx.close()

@@ -194,3 +194,23 @@ def test_gather_and_expansion_errors(xt):
     sz = ctx.packed_sizes()
     assert sz["max_d1"] == 9 * 2 and sz["max_d2"] == 9 * 2  # alpha tiles per space
     ctx.close()
+
+
+def test_sum_task_energies_order_and_errors(xt):
+    """CPU: xt_sum_task_energies adds in the fixed order of the device
+    reduction (256 runs of ceil(n/256) consecutive tasks, then a halving
+    tree).  For four tasks that pairs (0, 2) with (1, 3): [1e16, 1, -1e16, 1]
+    gives (1e16 - 1e16) + (1 + 1) = 2, where a serial sum gives 1 (1e16 + 1
+    rounds to 1e16).  Empty input gives 0; bad arguments are refused."""
+    import ctypes as C
+    assert xt.sum_task_energies(np.zeros((0, 2))) == (0.0, 0.0)
+    p = np.array([[1e16, 1e16], [1.0, 1.0], [-1e16, -1e16], [1.0, 1.0]])
+    assert xt.sum_task_energies(p) == (2.0, 2.0)
+    assert ((1e16 + 1.0) - 1e16) + 1.0 == 1.0
+    r = np.random.default_rng(5).normal(size=(10007, 2))
+    e4, e5 = xt.sum_task_energies(r)
+    assert abs(e4 - r[:, 0].sum()) < 1e-10 and abs(e5 - r[:, 1].sum()) < 1e-10
+    e = np.zeros(2)
+    ptr = e.ctypes.data_as(C.POINTER(C.c_double))
+    assert xt.lib().xt_sum_task_energies(-1, ptr, ptr) != 0
+    assert xt.lib().xt_sum_task_energies(3, None, ptr) != 0
