@@ -534,12 +534,87 @@ int xt_synth_closed_shell(int O, int V, uint64_t seed, double sigma, double t1_s
   return synth(O, V, seed, sigma, t1_sigma, eps_occ, eps_vir, t1, t2, vvoo, ovoo, vvvo);
 }
 
+namespace {
+// Device bytes of the closed-shell inputs of a tiling, following the
+// allocations of build_operands, build_xva and build_split: the resident FP64
+// operands, what an upload holds beyond them until the layout is built
+// (nchol = 0: the closed-shell upload; nchol > 0: the Cholesky upload, whose
+// high point is the largest of its GEMM, layout and xva steps), and the FP16
+// split copies of the mixed form.  Per-task arrays (bytes per task) are left
+// out.
+struct MemReq {
+  double operands, staging, split;
+};
+MemReq mem_req(size_t O, size_t V, long nchol) {
+  const double o = (double) O, v = (double) V;
+  const double op = (double) ((O + 7) / 8 * 8), vp = (double) ((V + 7) / 8 * 8);
+  const double ko = (double) ((O + 15) / 16 * 16), kv = (double) ((V + 15) / 16 * 16);
+  MemReq       r;
+  r.operands = 8 * (2 * v * v * o * op + 2 * o * o * v * op + 2 * v * v * o * vp + 2 * o * o * v * vp +
+                    v * v * o * o + v * o + o + v);
+  r.split    = 4 * (2 * v * v * o * ko + 2 * o * o * v * ko + 2 * v * v * o * kv + 2 * o * o * v * kv) +
+            2 * std::max(v * v * o, o * o * v);
+  const double t2 = 8 * v * v * o * o, ov = 8 * o * o * o * v, xv = 8 * v * v * o * vp;
+  if(nchol <= 0) {
+    r.staging = t2 + ov + 8 * v * v * v * o;  // t2, ovoo, vvvo as uploaded
+  }
+  else {
+    const double nmo = o + v, k = (double) nchol;
+    const double gemm = 8 * (v * v * o * o + v * o + o + v) + xv + t2 + 8 * nmo * nmo * k + ov +
+                        8 * (2 * v * o * k + o * o * k + v * v * o * o);
+    const double layout = r.operands - xv + t2 + ov + 8 * v * v * o * o;
+    const double xva    = r.operands + t2 + ov;
+    r.staging           = std::max({gemm, layout, xva}) - r.operands;
+  }
+  return r;
+}
+
+// Fails with XT_ENOMEM before anything is allocated when `need` bytes exceed
+// the device's free memory (hybrid.cpp:19-41 checks the (T) requirement
+// against the device's total memory the same way).
+int check_free(const std::string& what, double need, const std::string& parts) {
+  size_t fr = 0, tot = 0;
+  if(hipMemGetInfo(&fr, &tot) != hipSuccess) return XT_OK;  // no figure: hipMalloc reports
+  if(need <= (double) fr) return XT_OK;
+  const double gib = 1024.0 * 1024.0 * 1024.0;
+  char         buf[320];
+  snprintf(buf, sizeof buf,
+           "%s: GPU memory not sufficient for this tiling: %.1f GiB needed (%s), %.1f GiB free "
+           "of %.1f GiB",
+           what.c_str(), need / gib, parts.c_str(), fr / gib, tot / gib);
+  return fail(XT_ENOMEM, buf);
+}
+
+std::string gib_str(double a, const char* na, double b, const char* nb) {
+  char buf[160];
+  snprintf(buf, sizeof buf, "%.1f GiB %s + %.1f GiB %s", a / (1024.0 * 1024 * 1024), na,
+           b / (1024.0 * 1024 * 1024), nb);
+  return buf;
+}
+}  // namespace
+
+int xt_memory_required(xt_ctx* c, long nchol, double* bytes) {
+  if(!c || !bytes) return fail(XT_EINVAL, "xt_memory_required: null argument");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_memory_required: call xt_set_tiling first");
+  const MemReq r = mem_req(c->plan.O, c->plan.V, nchol);
+  bytes[0]       = r.operands;
+  bytes[1]       = r.staging;
+  bytes[2]       = r.split;
+  return XT_OK;
+}
+
 static int upload_common(xt_ctx* c, const double* eo, const double* ev, const double* t1,
                          const double* t2, const double* vvoo, const double* ovoo,
                          const double* vvvo) {
   const size_t o = c->plan.O, v = c->plan.V;
   XT_HIP(hipSetDevice(c->device));
   c->free_inputs();
+  {
+    const MemReq r  = mem_req(o, v, 0);
+    const int    rc = check_free("xt_upload_closed_shell", r.operands + r.staging,
+                                 gib_str(r.operands, "operands", r.staging, "upload staging"));
+    if(rc) return rc;
+  }
   hipStream_t st = c->stream;
   double *    t2_d = nullptr, *ovoo_d = nullptr, *vvvo_d = nullptr;
   auto        cleanup = [&]() {
@@ -588,6 +663,12 @@ int xt_upload_cholesky(xt_ctx* c, const double* eps_occ, const double* eps_vir, 
     return fail(XT_EINVAL, "xt_upload_cholesky: nmo * nchol beyond the BLAS 32-bit leading dimension");
   XT_HIP(hipSetDevice(c->device));
   c->free_inputs();
+  {
+    const MemReq r  = mem_req(o, v, nchol);
+    const int    rc = check_free("xt_upload_cholesky", r.operands + r.staging,
+                                 gib_str(r.operands, "operands", r.staging, "upload staging"));
+    if(rc) return rc;
+  }
   hipStream_t st = c->stream;
   double *    t2_d = nullptr, *ovoo_d = nullptr, *L_d = nullptr, *none = nullptr;
   auto        cleanup = [&]() {
@@ -938,6 +1019,12 @@ static int check_run_state(xt_ctx* c, const xt_opts& o) {
       for(int i = 0; i < P.k_range[t]; i++) to[P.spatial_off[t] + i] = t;
     for(int t = 0; t < P.nvab / 2; t++)
       for(int i = 0; i < P.k_range[P.noab + t]; i++) tv[P.spatial_off[P.noab + t] + i] = t;
+    {
+      const MemReq r  = mem_req(P.O, P.V, 0);
+      const int    rc = check_free("xt_run (mixed precision)", r.split,
+                                   gib_str(r.split, "FP16 split copies", 0.0, "more"));
+      if(rc) return rc;
+    }
     int err = xt::build_split(c->in, P.O, P.V, to.data(), P.noab / 2, tv.data(), P.nvab / 2,
                               c->split, c->stream);
     if(err) {

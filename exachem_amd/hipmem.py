@@ -27,12 +27,20 @@ def rt():
         _rt.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
         _rt.hipEventSynchronize.argtypes = [C.c_void_p]
         _rt.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        _rt.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     return _rt
 
 
 def _ok(rc, what):
     if rc != 0:
         raise RuntimeError(f"{what} failed: hipError {rc}")
+
+
+def mem_info():
+    """(free, total) device memory in bytes."""
+    fr, tot = C.c_size_t(), C.c_size_t()
+    _ok(rt().hipMemGetInfo(C.byref(fr), C.byref(tot)), "hipMemGetInfo")
+    return fr.value, tot.value
 
 
 class DevBuf:

@@ -224,6 +224,14 @@ class CCSDTriples:
         check(lib().xt_flops(self._ctx, _dp(per), C.byref(tot), _dp(parts)))
         return tot.value, per[:n], parts
 
+    def memory_required(self, nchol=0):
+        """Device bytes of the inputs of the current tiling: dict with the
+        resident FP64 operands, the upload staging (closed-shell upload, or
+        the Cholesky upload with nchol vectors) and the FP16 split copies."""
+        b = np.zeros(3)
+        check(lib().xt_memory_required(self._ctx, nchol, _dp(b)))
+        return {"operands": b[0], "staging": b[1], "split": b[2]}
+
     def upload(self, s):
         arrs = [np.ascontiguousarray(s[k], dtype=np.float64)
                 for k in ("eo", "ev", "t1", "t2", "vvoo", "ovoo", "vvvo")]

@@ -189,6 +189,16 @@ int xt_run(xt_ctx* ctx, const xt_opts* opts, double* e45, double* task_partials,
 int xt_run_tasks(xt_ctx* ctx, const xt_opts* opts, long n, const long* which, double* e45,
                  double* task_partials, float* task_ms);
 
+/* Device memory the inputs of the current tiling need, in bytes: bytes[0]
+ * the resident FP64 operand arrays, bytes[1] what an upload holds beyond them
+ * until the layout is built (nchol = 0: xt_upload_closed_shell; nchol > 0:
+ * xt_upload_cholesky with that many Cholesky vectors), bytes[2] the FP16
+ * split copies the first mixed-precision run adds.  Host only.  The uploads
+ * and the first mixed run check these figures against the device's free
+ * memory before allocating and fail with XT_ENOMEM and the figures
+ * otherwise; replaces check_memory_req (hybrid.cpp:19-41). */
+int xt_memory_required(xt_ctx* ctx, long nchol, double* bytes);
+
 /* E[4], E[5] from per-task contributions: task_partials[2*n] in task-list
  * order, factor included, as xt_run returns them.  The sum follows exactly
  * the fixed order of xt_run's device reduction.  A task's contribution is an

@@ -306,3 +306,25 @@ def test_resume_from_partials_is_bit_identical(xt):
         assert np.array_equal(parts.view(np.uint64), full["task_partials"].view(np.uint64))
         assert xt.sum_task_energies(parts) == (full["e4"], full["e5"])
     ctx.close()
+
+
+def test_memory_required_matches_allocations(xt):
+    """The device-memory figures of xt_memory_required against the memory an
+    upload and the first mixed run really take (hipMemGetInfo), O=24, V=160:
+    within 2% + 64 MiB (allocation granularity and per-task arrays)."""
+    from exachem_amd.hipmem import mem_info
+    ctx = xt.CCSDTriples(0)
+    ctx.set_tiling(xt.Tiling.uniform(24, 160, 8, 32))
+    req = ctx.memory_required()
+    print(f"[gpu] memory required O=24 V=160: {req}", flush=True)
+    f0, total = mem_info()
+    ctx.synth_upload(20241202)
+    f1, _ = mem_info()
+    ctx.prepare(xt.PREC_MIXED)
+    f2, _ = mem_info()
+    used_upload, used_split = f0 - f1, f1 - f2
+    print(f"[gpu] measured: upload {used_upload} B, split {used_split} B", flush=True)
+    assert abs(used_upload - req["operands"]) <= 0.02 * req["operands"] + 64 * 2**20
+    assert abs(used_split - req["split"]) <= 0.02 * req["split"] + 64 * 2**20
+    assert req["operands"] + req["staging"] < total
+    ctx.close()

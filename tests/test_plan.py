@@ -214,3 +214,22 @@ def test_sum_task_energies_order_and_errors(xt):
     ptr = e.ctypes.data_as(C.POINTER(C.c_double))
     assert xt.lib().xt_sum_task_energies(-1, ptr, ptr) != 0
     assert xt.lib().xt_sum_task_energies(3, None, ptr) != 0
+
+
+def test_memory_required(xt):
+    """CPU: the device-memory figures the uploads check before allocating
+    (in place of check_memory_req, hybrid.cpp:19-41) at O=48, V=384 and
+    O=64, V=512: resident FP64 operands, upload staging and FP16 split copies
+    in GB; the GPU test compares them with the memory an upload really takes.
+    No tiling -> refused."""
+    ctx = xt.CCSDTriples(-1)
+    with pytest.raises(xt.XtError):
+        ctx.memory_required()
+    for (O, V), want in (((48, 384), (57.8, 24.8, 27.5)), ((64, 512), (182.5, 78.4, 87.0))):
+        ctx.set_tiling(xt.Tiling.uneven(O, V, (6, 20), (16, 48), seed=1))
+        r = ctx.memory_required()
+        got = (r["operands"] / 1e9, r["staging"] / 1e9, r["split"] / 1e9)
+        assert np.allclose(got, want, atol=0.05), (O, V, got)
+        # the Cholesky upload stages L (nmo^2 K) instead of vvvo (V^3 O)
+        assert ctx.memory_required(nchol=4 * (O + V))["staging"] < r["staging"]
+    ctx.close()
