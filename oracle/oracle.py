@@ -22,8 +22,10 @@ What each function restates and how it is pinned:
   energies need SCF/CD/CCSD, which cannot be reproduced here).  Pinned by:
   a dense tile-free brute force of the antisymmetrised s1/d1/d2 expressions,
   tile invariance, restricted == unrestricted on closed-shell inputs,
-  T1 = 0 => E[5] = E[4], scaling laws, the MP4 triples identity and the
-  independent spatial formula (tests/test_oracle.py).  Energy sums are carried
+  T1 = 0 => E[5] = E[4], scaling laws, the MP4 triples identity, the
+  independent spatial formula, and Pauli-forbidden tasks (two holes or two
+  particles in one one-orbital tile) vanishing to FP64 rounding while every
+  other task does not (tests/test_oracle.py).  Energy sums are carried
   in long double (the reference uses double), so the values are the correctly
   rounded sums of the per-element terms.
 * ``task_packed`` -- the packed buffers handed to the GPU launcher
