@@ -569,27 +569,20 @@ MemReq mem_req(size_t O, size_t V, long nchol) {
   return r;
 }
 
-// Fails with XT_ENOMEM before anything is allocated when `need` bytes exceed
-// the device's free memory (hybrid.cpp:19-41 checks the (T) requirement
-// against the device's total memory the same way).
-int check_free(const std::string& what, double need, const std::string& parts) {
+// Fails with XT_ENOMEM before anything is allocated when a + b bytes (named
+// na, nb) exceed the device's free memory (hybrid.cpp:19-41 checks the (T)
+// requirement against the device's total memory the same way).
+int check_free(const char* what, double a, const char* na, double b, const char* nb) {
   size_t fr = 0, tot = 0;
   if(hipMemGetInfo(&fr, &tot) != hipSuccess) return XT_OK;  // no figure: hipMalloc reports
-  if(need <= (double) fr) return XT_OK;
+  if(a + b <= (double) fr) return XT_OK;
   const double gib = 1024.0 * 1024.0 * 1024.0;
-  char         buf[320];
+  char         buf[384];
   snprintf(buf, sizeof buf,
-           "%s: GPU memory not sufficient for this tiling: %.1f GiB needed (%s), %.1f GiB free "
-           "of %.1f GiB",
-           what.c_str(), need / gib, parts.c_str(), fr / gib, tot / gib);
+           "%s: GPU memory not sufficient for this tiling: %.1f GiB needed (%.1f GiB %s + %.1f "
+           "GiB %s), %.1f GiB free of %.1f GiB",
+           what, (a + b) / gib, a / gib, na, b / gib, nb, fr / gib, tot / gib);
   return fail(XT_ENOMEM, buf);
-}
-
-std::string gib_str(double a, const char* na, double b, const char* nb) {
-  char buf[160];
-  snprintf(buf, sizeof buf, "%.1f GiB %s + %.1f GiB %s", a / (1024.0 * 1024 * 1024), na,
-           b / (1024.0 * 1024 * 1024), nb);
-  return buf;
 }
 }  // namespace
 
@@ -611,8 +604,8 @@ static int upload_common(xt_ctx* c, const double* eo, const double* ev, const do
   c->free_inputs();
   {
     const MemReq r  = mem_req(o, v, 0);
-    const int    rc = check_free("xt_upload_closed_shell", r.operands + r.staging,
-                                 gib_str(r.operands, "operands", r.staging, "upload staging"));
+    const int    rc =
+      check_free("xt_upload_closed_shell", r.operands, "operands", r.staging, "upload staging");
     if(rc) return rc;
   }
   hipStream_t st = c->stream;
@@ -665,8 +658,8 @@ int xt_upload_cholesky(xt_ctx* c, const double* eps_occ, const double* eps_vir, 
   c->free_inputs();
   {
     const MemReq r  = mem_req(o, v, nchol);
-    const int    rc = check_free("xt_upload_cholesky", r.operands + r.staging,
-                                 gib_str(r.operands, "operands", r.staging, "upload staging"));
+    const int    rc =
+      check_free("xt_upload_cholesky", r.operands, "operands", r.staging, "upload staging");
     if(rc) return rc;
   }
   hipStream_t st = c->stream;
@@ -1021,8 +1014,8 @@ static int check_run_state(xt_ctx* c, const xt_opts& o) {
       for(int i = 0; i < P.k_range[P.noab + t]; i++) tv[P.spatial_off[P.noab + t] + i] = t;
     {
       const MemReq r  = mem_req(P.O, P.V, 0);
-      const int    rc = check_free("xt_run (mixed precision)", r.split,
-                                   gib_str(r.split, "FP16 split copies", 0.0, "more"));
+      const int    rc =
+        check_free("xt_run (mixed precision)", r.split, "FP16 split copies", 0.0, "other");
       if(rc) return rc;
     }
     int err = xt::build_split(c->in, P.O, P.V, to.data(), P.noab / 2, tv.data(), P.nvab / 2,
