@@ -10,16 +10,29 @@ distributions (integrals N(0, 0.05^2), T1 N(0, 1e-6), MP2-like T2), mixed
 precision.
 
 Prints ONE JSON line.  value = time-to-E(T) in seconds (lower is better).
+A wall-clock budget (--budget-s, counted from process start) caps the number
+of warm-up and timed steps: at O=48, V=384 one step is about two minutes, so
+the timed steps are as many of those requested as fit after the bounded legs
+below, at least one; the line reports steps and warm-up requested beside those
+run.  The calibration launches (autotuner, cost model, comparison sample) run
+the same kernel and graph before the timed steps.
 roofline: the reference's analytic flop count per launch over the persistent
 kernel's HIP-event time.  Beside it:
   * fp64_compare: the plain FP64 kernel and the measured precision on the same
     seeded task sample (or on the whole workload with --fp64-full): time ratio
     and |dE4|, |dE5| of the measured precision against FP64;
+  * schedule_ab: the measured precision on the same comparison sample under
+    the reference's driver order, the analytic flop-count schedule and the
+    learned schedule (same kernel, same group shape);
+  * oracle_check: GPU FP64 and the measured precision, task by task, against
+    the oracle's energies committed in tests/golden for this workload's seeded
+    inputs (written on the CPU by tests/golden/make_golden.py), and the same at
+    a physical energy scale (integrals scaled so that |E(T)| is below 1 Eh);
   * cpu_baseline: the CPU oracle (a C++ restatement of the reference's CPU
-    fused (T) path) timed on a bounded sample of the workload's tasks,
-    extrapolated by the flop count; its energies on those tasks are the
-    reference values for oracle_check (GPU FP64 and measured precision vs the
-    oracle, task by task).
+    fused (T) path) timed on a seeded, size-stratified task sample (two
+    tasks per flop quartile, smallest first, until its budget is spent),
+    extrapolated by the flop count; its energies on those tasks are checked
+    against the GPU too (cpu_sample_check).
 """
 import argparse
 import ctypes as C
@@ -30,6 +43,7 @@ import time
 
 import numpy as np
 
+T_START = time.time()
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -56,19 +70,40 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def profiled_traffic(config, O_, V_, precision):
-    """HBM bytes per launch of this kernel and workload from the committed
-    rocprofv3 summary (two --pmc passes, calibrated), or (None, None)."""
-    import re
-    kern = {"mixed": "k_spatial_mixed", "fp64": "k_spatial_fp64"}[precision]
-    path = os.path.join(ROOT, "profiles", f"{kern}_{config}_O{O_}_V{V_}.md")
-    if not os.path.exists(path):
-        return None, None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", open(path).read())
-    return (float(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
+def elapsed():
+    return time.time() - T_START
 
 
-def gather_leg(xt, ctx, tiling, s, n_tasks, cpu=True):
+def golden_case(name):
+    path = os.path.join(ROOT, "tests", "golden", name)
+    return json.load(open(path)) if os.path.exists(path) else None
+
+
+def fixture_check(xt, ctx, case, precisions):
+    """Every task of a committed oracle fixture run on the GPU at each
+    precision; max abs/rel differences of E4, E5 against the oracle, and
+    whether FP64 is within 1e-10 Eh + 2e-14 |E| on every task."""
+    which = np.asarray(case["which"], dtype=np.int64)
+    ref = np.asarray(case["task_partials"])
+    out = {"tasks": int(len(which)), "roles": case.get("roles"), "sigma": case["sigma"],
+           "reference": "oracle (CPU restatement of the reference path), committed fixture "
+                        f"tests/golden/{case['file']}",
+           "E4_sum_oracle": float(ref[:, 0].sum()), "E5_sum_oracle": float(ref[:, 1].sum())}
+    for name, p in precisions:
+        g = ctx.run_tasks(which, precision=p)["task_partials"]
+        d = np.abs(g - ref)
+        r = {"max_abs_dE4": float(d[:, 0].max()), "max_abs_dE5": float(d[:, 1].max()),
+             "max_rel_dE4": float(np.max(d[:, 0] / np.abs(ref[:, 0]))),
+             "max_rel_dE5": float(np.max(d[:, 1] / np.abs(ref[:, 1]))),
+             "abs_dE4_sum": float(abs(g[:, 0].sum() - ref[:, 0].sum())),
+             "abs_dE5_sum": float(abs(g[:, 1].sum() - ref[:, 1].sum()))}
+        if p == xt.PREC_FP64:
+            r["within_1e-10_plus_2e-14_rel"] = bool(np.all(d <= 1e-10 + 2e-14 * np.abs(ref)))
+        out[name] = r
+    return out
+
+
+def gather_leg(xt, ctx, tiling, s, n_tasks, cpu_tasks=4):
     """The per-task path's gather (ccsd_t_data_{s1,d1,d2}_new) on the device
     (xt_gather_task) for a seeded sample of the workload's tasks, timed with
     HIP events on its stream; beside it the oracle's restatement of the host
@@ -86,13 +121,14 @@ def gather_leg(xt, ctx, tiling, s, n_tasks, cpu=True):
     st, e0, e1 = Stream(), Event(), Event()
     ctx.gather_task(tasks[pick[0]], ptrs, st.h)  # warm-up (first launch)
     st.sync()
-    gpu_ms, elems = 0.0, 0
-    for k in pick:
+    gpu_ms_task, elems = np.zeros(len(pick)), 0
+    for i, k in enumerate(pick):
         e0.record(st.h)
         ctx.gather_task(tasks[k], ptrs, st.h)
         e1.record(st.h)
-        gpu_ms += e1.elapsed_ms(e0)
+        gpu_ms_task[i] = e1.elapsed_ms(e0)
         elems += ctx.gather_last_elems()
+    gpu_ms = float(gpu_ms_task.sum())
     for b in dev.values():
         b.free()
     for x in (st, e0, e1):
@@ -102,19 +138,22 @@ def gather_leg(xt, ctx, tiling, s, n_tasks, cpu=True):
            "roofline": {"bound": "hbm", "achieved": 16 * elems / gpu_ms / 1e6, "peak": 8000.0,
                         "unit": "GB/s", "frac": 16 * elems / gpu_ms / 1e6 / 8000.0,
                         "bytes_per_double": "8 written + 8 read (lower bound)"}}
-    if cpu:
+    if cpu_tasks > 0:
         from oracle import oracle as O
         O.build()
         occ, vir = tiling.occ, tiling.vir
         g = O.SpatialGather(O.Tiling(occ, occ, vir, vir), s)
+        sub = pick[:cpu_tasks]
         t0, n_cpu = time.perf_counter(), 0
-        for k in pick:
+        for k in sub:
             n_cpu += g(tasks[k])
         cpu_ms = (time.perf_counter() - t0) * 1e3
+        gpu_sub = gpu_ms_task[:len(sub)].sum()
+        out["cpu_tasks"] = int(len(sub))
         out["cpu_ms"] = cpu_ms
         out["cpu_threads"] = 1
         out["cpu_doubles_written"] = int(n_cpu)
-        out["speedup_vs_cpu"] = cpu_ms / gpu_ms
+        out["speedup_vs_cpu"] = cpu_ms / gpu_sub
     return out
 
 
@@ -251,32 +290,41 @@ def cholesky_leg(xt, tiling, kfac, cpu=True):
     return out
 
 
-def cpu_baseline(tiling, s, per_task, budget_s):
-    """Times the oracle on tasks of the workload, from the 25th flop
-    percentile upwards, until the budget is spent."""
+def cpu_baseline(tiling, s, per_task, budget_s, n_per_quartile=2):
+    """Times the oracle on a seeded, size-stratified sample of the workload's
+    tasks: n_per_quartile tasks drawn from each flop quartile, run smallest
+    first until the budget is spent (at least one task)."""
     from oracle import oracle as O
     O.build()
     occ, vir = tiling.occ, tiling.vir
     t = O.Tiling(occ, occ, vir, vir)
     evl = np.concatenate([s["eo"], s["eo"], s["ev"], s["ev"]])
     order = np.argsort(per_task, kind="stable")
-    pick, parts, spent, flops = [], [], 0.0, 0.0
-    for k in order[len(order) // 4:]:
+    rng = np.random.default_rng(SEED)
+    strata = np.array_split(order, 4)
+    sample = np.concatenate([rng.choice(q, min(n_per_quartile, len(q)), replace=False)
+                             for q in strata])
+    sample = sample[np.argsort(per_task[sample], kind="stable")]
+    pick, parts, secs = [], [], []
+    for k in sample:
+        if pick and sum(secs) > budget_s:
+            break
         t0 = time.time()
         _, _, pt = O.run_spatial(t, evl, s, True, [int(k)])
-        spent += time.time() - t0
-        flops += per_task[k]
+        secs.append(time.time() - t0)
         pick.append(int(k))
         parts.append(pt[0])
-        if spent > budget_s:
-            break
+    spent, flops = float(sum(secs)), float(per_task[pick].sum())
     model = "unknown CPU"
     try:
         with open("/proc/cpuinfo") as f:
             model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
     except (OSError, StopIteration):
         pass
-    return dict(rate=flops / spent, spent=spent, pick=pick, flops=flops,
+    quart = [int(np.searchsorted(np.sort(per_task), per_task[k], side="right") * 4 //
+                 (len(per_task) + 1)) for k in pick]
+    return dict(rate=flops / spent, spent=spent, pick=pick, flops=flops, sampled=len(sample),
+                task_flops=[float(per_task[k]) for k in pick], task_s=secs, quartiles=quart,
                 threads=O.num_threads(), partials=np.asarray(parts), model=model,
                 nproc=os.cpu_count())
 
@@ -286,29 +334,38 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--budget-s", type=float, default=500.0,
+                    help="wall-clock budget of the whole run from process start; warm-up and "
+                         "timed steps are capped to fit it (at least one timed step runs)")
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--schedule", default="learned", choices=["driver", "analytic", "learned"])
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default ticket group shape instead of tuning it")
-    ap.add_argument("--tune-sample", type=int, default=256,
+    ap.add_argument("--tune-sample", type=int, default=128,
                     help="tasks timed per candidate by the autotuner")
     ap.add_argument("--learn-sample", type=int, default=256,
                     help="tasks timed per precision to train the cost model")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of the graph")
     ap.add_argument("--task-sample", type=int, default=0,
                     help="profiling aid: time only this many tasks (not the benchmark workload)")
-    ap.add_argument("--compare-frac", type=float, default=0.04,
+    ap.add_argument("--compare-frac", type=float, default=0.02,
                     help="fraction of tasks in the seeded FP64 comparison sample")
     ap.add_argument("--fp64-full", action="store_true",
                     help="compare against a full FP64 run instead of a task sample")
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--no-schedule-ab", action="store_true",
+                    help="skip timing the driver-order and analytic schedules on the sample")
     ap.add_argument("--sigma", type=float, default=0.05,
                     help="standard deviation of the synthetic integrals (E4 scales as sigma^4)")
     ap.add_argument("--t1-sigma", type=float, default=1e-3,
                     help="standard deviation of the synthetic T1 amplitudes")
+    ap.add_argument("--no-physical", action="store_true",
+                    help="skip the physical-scale check against the committed oracle fixture")
     ap.add_argument("--gather-tasks", type=int, default=16,
                     help="tasks of the device-side gather leg (0 skips it)")
+    ap.add_argument("--gather-cpu-tasks", type=int, default=4,
+                    help="of those, tasks also gathered by the oracle on one host core")
     ap.add_argument("--dropin-tasks", type=int, default=8,
                     help="tasks of the per-task drop-in leg: device gather plus "
                          "xt_fully_fused_ccsd_t_gpu (0 skips it)")
@@ -317,7 +374,7 @@ def main():
                          "GPU (xt_upload_cholesky) against numpy on the host cores; 0 skips it")
     ap.add_argument("--expand-leg", action="store_true",
                     help="also time the dense spin-orbital T1/T2 expansion (xt_expand_amplitudes)")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -328,8 +385,9 @@ def main():
     import exachem_amd as xt
     PREC = {"fp64": xt.PREC_FP64, "mixed": xt.PREC_MIXED}
     prec = PREC[args.precision]
-    sched = {"driver": xt.SCHED_DRIVER, "analytic": xt.SCHED_ANALYTIC,
-             "learned": xt.SCHED_LEARNED}[args.schedule]
+    SCHED = {"driver": xt.SCHED_DRIVER, "analytic": xt.SCHED_ANALYTIC,
+             "learned": xt.SCHED_LEARNED}
+    sched = SCHED[args.schedule]
     graph = 0 if args.no_graph else 1
     tiling = make_tiling(xt, args.config)
     O_, V_ = tiling.O, tiling.V
@@ -345,16 +403,15 @@ def main():
     t0 = time.time()
     ctx.prepare(prec)
     operand_s = time.time() - t0
-    # one-time calibration (untimed, like the upload): tile-shape autotuning
-    # and the learned cost model
+    # one-time calibration (untimed, like the upload): tile-shape autotuning of
+    # the measured precision and the learned cost model
     tune = {}
     t0 = time.time()
     if not args.no_autotune:
-        for name, p in (("fp64", xt.PREC_FP64), (args.precision, prec)):
-            if name in tune:
-                continue
-            if p != prec:
-                ctx.prepare(p)
+        tune_precs = [(args.precision, prec)]
+        if args.fp64_full and args.precision != "fp64":
+            tune_precs.append(("fp64", xt.PREC_FP64))
+        for name, p in tune_precs:
             best, times, cands = ctx.autotune(precision=p, n_sample=args.tune_sample)
             tune[name] = {"shape": cands[best], "candidates": cands,
                           "sample_ms": [round(x, 2) for x in times]}
@@ -384,27 +441,9 @@ def main():
         return ctx.run_tasks(which, precision=prec, schedule=sched, use_graph=graph,
                              progress_ms=30000)
 
-    for w in range(args.warmup):
-        step()
-        log(f"warmup {w} done")
-    times, kms, res = [], [], None
-    for k in range(args.steps):
-        t0 = time.perf_counter()
-        res = step()
-        times.append(time.perf_counter() - t0)
-        kms.append(res["kernel_ms"])
-        log(f"step {k}: {times[-1]:.3f}s kernel {res['kernel_ms']:.1f} ms "
-            f"E4={res['e4']!r} E5={res['e5']!r}")
-    sec = float(np.mean(times))
-    kernel_s = float(np.mean(kms)) / 1e3
-    achieved = total_flops / kernel_s / 1e12
-    peak = PEAK_TFLOPS[args.precision]
-    traffic, traffic_src = (None, None) if which is not None else profiled_traffic(
-        args.config, O_, V_, args.precision)
     line = {
         "metric": "time-to-E(T) + MFMA TFLOP/s at O=48,V=384; |dE(T)| vs FP64 oracle",
-        "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "strong",
+        "unit": "s", "n_gpus": 1, "higher_is_better": False, "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f16x3 products / f32 accum / f64 energy",
         "data": "synthetic (seeded generator, stated distributions)",
@@ -416,21 +455,11 @@ def main():
                    "schedule": args.schedule, "graph": bool(graph),
                    "sigma": args.sigma, "t1_sigma": args.t1_sigma,
                    "parallelism": "single GPU, persistent kernel over per-XCD work queues"},
-        "E4": res["e4"], "E5": res["e5"], "tflops": achieved,
-        "setup_s": {"upload_and_layout": upload_s, "operand_split": operand_s,
-                    "autotune": tune_s, "learn_schedule": learn_s},
-        "autotune": tune or None,
-        "learned_schedule": learn,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
-                     "flops_per_launch": total_flops, "flop_count": "reference analytic count",
-                     "kernel_ms": kernel_s * 1e3},
     }
-    if args.precision == "mixed":
-        # three FP16 MFMA products per d1/d2 multiply-add
-        issued = (3 * (parts[1] + parts[2]) + parts[0]) if which is None else 3 * total_flops
-        line["roofline"]["mfma_issued_tflops"] = issued / kernel_s / 1e12
 
+    # FP64 against the measured precision on a seeded task sample; its mixed
+    # run under the chosen schedule also gives the estimate of one full step
+    est_step = None
     if not args.no_compare and args.precision != "fp64":
         if args.fp64_full:
             cw = np.arange(n_tasks)
@@ -443,9 +472,12 @@ def main():
                           progress_ms=30000)
         m = ctx.run_tasks(cw, precision=prec, schedule=sched, use_graph=graph, progress_ms=30000)
         d4, d5 = abs(m["e4"] - f["e4"]), abs(m["e5"] - f["e5"])
+        cflops = float(per_task[cw].sum())
+        est_step = m["kernel_ms"] / 1e3 * total_flops / cflops
         line["fp64_compare"] = {
-            "tasks": int(len(cw)), "full": bool(args.fp64_full),
-            "flops": float(per_task[cw].sum()),
+            "tasks": int(len(cw)), "full": bool(args.fp64_full), "flops": cflops,
+            "reference": "GPU FP64 kernel on the same tasks (FP64 matches the oracle to "
+                         "1e-10 Eh + 2e-14 |E| per task, see oracle_check)",
             "fp64_kernel_ms": f["kernel_ms"], "kernel_ms": m["kernel_ms"],
             "speedup_vs_fp64": f["kernel_ms"] / m["kernel_ms"],
             "E4_fp64": f["e4"], "E5_fp64": f["e5"],
@@ -457,6 +489,32 @@ def main():
         }
         log(f"fp64 {f['kernel_ms']:.1f} ms, {args.precision} {m['kernel_ms']:.1f} ms, "
             f"rel dE4 {d4 / abs(f['e4']):.2e}")
+        if not args.no_schedule_ab:
+            ab = {args.schedule: m["kernel_ms"]}
+            for name in ("driver", "analytic", "learned"):
+                if name not in ab and (name != "learned" or learn is not None):
+                    r = ctx.run_tasks(cw, precision=prec, schedule=SCHED[name], use_graph=graph)
+                    ab[name] = r["kernel_ms"]
+                    if r["e4"] != m["e4"] or r["e5"] != m["e5"]:
+                        raise AssertionError(f"schedule {name} changed the energies")
+            line["schedule_ab"] = {
+                "tasks": int(len(cw)), "precision": args.precision,
+                "kernel_ms": ab, "energies_bit_identical": True,
+                "learned_vs_analytic": ab.get("analytic", np.nan) / ab.get("learned", np.nan),
+                "learned_vs_driver": ab.get("driver", np.nan) / ab.get("learned", np.nan)}
+            log(f"schedule A/B: {ab}")
+
+    # GPU against the oracle on the committed fixtures of this workload
+    fx = {"c3": ("parity_c3_sample.json", "parity_c3_phys_sample.json"),
+          "c4": ("parity_c4_sample.json", None)}.get(args.config, (None, None))
+    precs = [("fp64", xt.PREC_FP64)] + ([(args.precision, prec)] if args.precision != "fp64"
+                                         else [])
+    case = golden_case(fx[0]) if fx[0] else None
+    if (case and case["sigma"] == args.sigma and case["t1_sigma"] == args.t1_sigma
+            and case["occ_tiles"] == tiling.occ and case["vir_tiles"] == tiling.vir):
+        case["file"] = fx[0]
+        line["oracle_check"] = fixture_check(xt, ctx, case, precs)
+        log(f"oracle check: {line['oracle_check']}")
 
     if args.cholesky_leg > 0:
         line["cholesky_v2"] = cholesky_leg(xt, tiling, args.cholesky_leg,
@@ -469,7 +527,8 @@ def main():
 
     if args.gather_tasks > 0:
         line["device_gather"] = gather_leg(xt, ctx, tiling, s, args.gather_tasks,
-                                           cpu=not args.no_cpu_baseline)
+                                           cpu_tasks=0 if args.no_cpu_baseline
+                                           else args.gather_cpu_tasks)
         log(f"device gather: {line['device_gather']}")
 
     if args.dropin_tasks > 0:
@@ -481,19 +540,97 @@ def main():
         line["cpu_baseline"] = {
             "value": total_flops / cb["rate"], "unit": "s (extrapolated time-to-E(T))",
             "cores": cb["threads"], "kind": "port",
-            "sample": f"{len(cb['pick'])} tasks of this workload ({cb['flops']:.3e} flops) "
-                      f"timed in {cb['spent']:.1f}s = {cb['rate'] / 1e9:.2f} GFLOP/s on "
-                      f"{cb['threads']} OpenMP threads ({cb['model']}, nproc {cb['nproc']}), "
-                      f"extrapolated by the analytic flop count"}
+            "sample": f"{len(cb['pick'])} of {cb['sampled']} seeded size-stratified tasks "
+                      f"(two per flop quartile, smallest first; quartiles {cb['quartiles']}, "
+                      f"{min(cb['task_flops']):.2e}-{max(cb['task_flops']):.2e} flops each, "
+                      f"{cb['flops']:.3e} in all) timed in {cb['spent']:.1f}s = "
+                      f"{cb['rate'] / 1e9:.2f} GFLOP/s on {cb['threads']} OpenMP threads "
+                      f"({cb['model']}, nproc {cb['nproc']}), extrapolated by the analytic "
+                      f"flop count",
+            "task_gflops": [round(fl / sec / 1e9, 3) for fl, sec in
+                            zip(cb["task_flops"], cb["task_s"])]}
         chk = {}
-        for name, p in (("fp64", xt.PREC_FP64), (args.precision, prec)):
+        for name, p in precs:
             g = ctx.run_tasks(cb["pick"], precision=p)["task_partials"]
             ref = cb["partials"]
             chk[name] = {"max_abs_dE4": float(np.max(np.abs(g[:, 0] - ref[:, 0]))),
                          "max_abs_dE5": float(np.max(np.abs(g[:, 1] - ref[:, 1]))),
                          "max_rel_dE4": float(np.max(np.abs(g[:, 0] - ref[:, 0]) /
                                                      np.abs(ref[:, 0])))}
-        line["oracle_check"] = dict(tasks=cb["pick"], **chk)
+        line["cpu_sample_check"] = dict(tasks=cb["pick"], **chk)
+        log(f"cpu baseline: {line['cpu_baseline']}")
+    del s
+
+    # physical energy scale: the same tiling with integrals scaled down so
+    # that |E(T)| is below 1 Eh, against its committed oracle fixture
+    case = golden_case(fx[1]) if fx[1] and not args.no_physical else None
+    if case and case["occ_tiles"] == tiling.occ and case["vir_tiles"] == tiling.vir:
+        case["file"] = fx[1]
+        t0 = time.time()
+        pctx = xt.CCSDTriples(0)
+        pctx.set_tiling(tiling)
+        pctx.synth_upload(case["seed"], case["sigma"], case["t1_sigma"])
+        if args.precision != "fp64":
+            pctx.prepare(prec)
+        line["oracle_check_physical"] = fixture_check(xt, pctx, case, precs)
+        pctx.close()
+        line["oracle_check_physical"]["seconds"] = time.time() - t0
+        log(f"physical-scale check: {line['oracle_check_physical']}")
+
+    # timed steps: as many of those requested as fit the budget (at least one)
+    t_steps = elapsed()
+    if est_step is None:
+        est_step = max(1e-3, float(np.mean(
+            [ctx.run_tasks(np.arange(min(n_tasks, 64)), precision=prec)["kernel_ms"]])) / 1e3 *
+            total_flops / float(per_task[:min(n_tasks, 64)].sum()))
+    fit = int(max(0.0, args.budget_s - t_steps) // (1.05 * est_step + 0.5))
+    n_steps = min(args.steps, max(1, fit))
+    n_warm = min(args.warmup, max(0, fit - n_steps))
+    log(f"{t_steps:.0f}s elapsed; estimated step {est_step:.1f}s; budget {args.budget_s:.0f}s "
+        f"fits {fit}: warm-up {n_warm}/{args.warmup}, steps {n_steps}/{args.steps}")
+    for w in range(n_warm):
+        step()
+        log(f"warmup {w} done")
+    times, kms, res = [], [], None
+    for k in range(n_steps):
+        t0 = time.perf_counter()
+        res = step()
+        times.append(time.perf_counter() - t0)
+        kms.append(res["kernel_ms"])
+        log(f"step {k}: {times[-1]:.3f}s kernel {res['kernel_ms']:.1f} ms "
+            f"E4={res['e4']!r} E5={res['e5']!r}")
+        if k + 1 < n_steps and elapsed() + 1.05 * times[-1] > args.budget_s:
+            log("budget reached")
+            break
+    sec = float(np.mean(times))
+    kernel_s = float(np.mean(kms)) / 1e3
+    achieved = total_flops / kernel_s / 1e12
+    peak = PEAK_TFLOPS[args.precision]
+    line.update({
+        "value": sec, "steps": len(times), "warmup": n_warm, "ms_per_step": sec * 1e3,
+        "steps_requested": args.steps, "warmup_requested": args.warmup,
+        "step_times_s": [round(x, 3) for x in times],
+        "budget": {"budget_s": args.budget_s, "elapsed_before_steps_s": t_steps,
+                   "estimated_step_s": est_step,
+                   "warmup_note": "autotuner, cost-model and comparison launches of the same "
+                                  "kernel and graph precede the timed steps"},
+        "E4": res["e4"], "E5": res["e5"], "tflops": achieved,
+        "setup_s": {"upload_and_layout": upload_s, "operand_split": operand_s,
+                    "autotune": tune_s, "learn_schedule": learn_s},
+        "autotune": tune or None,
+        "learned_schedule": learn,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "traffic_note": "HBM bytes per launch come from rocprofv3 --pmc passes "
+                                     "outside this run (profiles/)",
+                     "flops_per_launch": total_flops, "flop_count": "reference analytic count",
+                     "kernel_ms": kernel_s * 1e3},
+    })
+    if args.precision == "mixed":
+        # three FP16 MFMA products per d1/d2 multiply-add
+        issued = (3 * (parts[1] + parts[2]) + parts[0]) if which is None else 3 * total_flops
+        line["roofline"]["mfma_issued_tflops"] = issued / kernel_s / 1e12
+    line["wall_s"] = elapsed()
     print(json.dumps(line), flush=True)
     ctx.close()
 

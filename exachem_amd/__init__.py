@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libexachem_t.so")
 HEADER = os.path.join(os.path.dirname(_HERE), "include", "exachem_t.h")
 
-XT_OK, XT_EINVAL, XT_ENOMEM, XT_EHIP, XT_ESTATE, XT_ENODEV = 0, -1, -2, -3, -4, -5
+XT_OK, XT_EINVAL, XT_ENOMEM, XT_EHIP, XT_ESTATE, XT_ENODEV, XT_ERANGE = 0, -1, -2, -3, -4, -5, -6
 PREC_FP64, PREC_MIXED = 0, 1
 SCHED_DRIVER, SCHED_ANALYTIC, SCHED_LEARNED = 0, 1, 2
 

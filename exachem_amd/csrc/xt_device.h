@@ -106,6 +106,8 @@ struct RunDev {
   const int*          qbeg;    // [kQueues+1]
   const long long*    qtotal;  // [kQueues]
   unsigned long long* qctr;    // [kQueues] ticket counters (zeroed before a run)
+  unsigned long long* err;     // [1] set by a box whose energy partial is not finite or
+                               // beyond 2^62 Eh (zeroed before a run, read after it)
   unsigned long long* acc;     // [4*n_tasks] (E4 lo, E4 hi, E5 lo, E5 hi): 128-bit
                                // two's-complement fixed point, scale 2^64 per Eh
   unsigned long long* cost;    // [n_tasks] summed workgroup wall-clock ticks (optional)

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -596,6 +597,31 @@ int xt_memory_required(xt_ctx* c, long nchol, double* bytes) {
   return XT_OK;
 }
 
+// First non-finite element of each host input, checked before any device
+// work (a NaN or an infinity would otherwise reach the energy sums).
+static int check_finite(const char* fn, std::initializer_list<std::pair<const char*, std::pair<const double*, size_t>>> arrays) {
+  for(const auto& a: arrays) {
+    const double* p = a.second.first;
+    const size_t  n = a.second.second;
+    const long    chunks = (long) ((n + (1u << 22) - 1) >> 22);
+    std::atomic<size_t> first{SIZE_MAX};
+    parallel_for(chunks, [&](long k) {
+      const size_t b = (size_t) k << 22, e = std::min(n, b + (1u << 22));
+      for(size_t i = b; i < e; i++)
+        if(!std::isfinite(p[i])) {
+          size_t cur = first.load();
+          while(i < cur && !first.compare_exchange_weak(cur, i)) {
+          }
+          break;
+        }
+    });
+    if(first.load() != SIZE_MAX)
+      return fail(XT_ERANGE, std::string(fn) + ": " + a.first + "[" + std::to_string(first.load()) +
+                               "] is not finite");
+  }
+  return XT_OK;
+}
+
 static int upload_common(xt_ctx* c, const double* eo, const double* ev, const double* t1,
                          const double* t2, const double* vvoo, const double* ovoo,
                          const double* vvvo) {
@@ -650,8 +676,15 @@ int xt_upload_cholesky(xt_ctx* c, const double* eps_occ, const double* eps_vir, 
   if(!c || !eps_occ || !eps_vir || !t1 || !t2 || !chol || nchol <= 0)
     return fail(XT_EINVAL, "xt_upload_cholesky: null argument or nchol <= 0");
   if(!c->have_plan) return fail(XT_ESTATE, "xt_upload_cholesky: call xt_set_tiling first");
-  if(c->device < 0) return fail(XT_ENODEV, "xt_upload_cholesky: host-only context");
   const size_t o = c->plan.O, v = c->plan.V, nmo = o + v;
+  {
+    const int rc = check_finite("xt_upload_cholesky",
+                                {{"eps_occ", {eps_occ, o}}, {"eps_vir", {eps_vir, v}},
+                                 {"t1", {t1, v * o}}, {"t2", {t2, v * v * o * o}},
+                                 {"chol", {chol, nmo * nmo * (size_t) nchol}}});
+    if(rc) return rc;
+  }
+  if(c->device < 0) return fail(XT_ENODEV, "xt_upload_cholesky: host-only context");
   if((long) (nmo * nchol) > (1l << 31) - 1 || (long) (v * o) > (1l << 31) - 1)
     return fail(XT_EINVAL, "xt_upload_cholesky: nmo * nchol beyond the BLAS 32-bit leading dimension");
   XT_HIP(hipSetDevice(c->device));
@@ -712,6 +745,15 @@ int xt_upload_closed_shell(xt_ctx* c, const double* eps_occ, const double* eps_v
   if(!c || !eps_occ || !eps_vir || !t1 || !t2 || !vvoo || !ovoo || !vvvo)
     return fail(XT_EINVAL, "xt_upload_closed_shell: null argument");
   if(!c->have_plan) return fail(XT_ESTATE, "xt_upload_closed_shell: call xt_set_tiling first");
+  {
+    const size_t o = c->plan.O, v = c->plan.V;
+    const int    rc = check_finite(
+      "xt_upload_closed_shell",
+      {{"eps_occ", {eps_occ, o}}, {"eps_vir", {eps_vir, v}}, {"t1", {t1, v * o}},
+       {"t2", {t2, v * v * o * o}}, {"vvoo", {vvoo, v * v * o * o}},
+       {"ovoo", {ovoo, o * v * o * o}}, {"vvvo", {vvvo, v * v * v * o}}});
+    if(rc) return rc;
+  }
   if(c->device < 0) return fail(XT_ENODEV, "xt_upload_closed_shell: host-only context");
   return upload_common(c, eps_occ, eps_vir, t1, t2, vvoo, ovoo, vvvo);
 }
@@ -719,6 +761,8 @@ int xt_upload_closed_shell(xt_ctx* c, const double* eps_occ, const double* eps_v
 int xt_synth_upload(xt_ctx* c, uint64_t seed, double sigma, double t1_sigma) {
   if(!c) return fail(XT_EINVAL, "xt_synth_upload: null context");
   if(!c->have_plan) return fail(XT_ESTATE, "xt_synth_upload: call xt_set_tiling first");
+  if(!std::isfinite(sigma) || !std::isfinite(t1_sigma))
+    return fail(XT_ERANGE, "xt_synth_upload: sigma and t1_sigma must be finite");
   if(c->device < 0) return fail(XT_ENODEV, "xt_synth_upload: host-only context");
   const size_t o = c->plan.O, v = c->plan.V;
   std::vector<double> eo(o), ev(v), t1(v * o);
@@ -773,7 +817,8 @@ static int ensure_plan_dev(xt_ctx* c) {
   XT_HIP(hipMalloc((void**) &c->d_qtick, sizeof(long long) * nt));
   XT_HIP(hipMalloc((void**) &c->d_qbeg, sizeof(int) * (xt::kQueues + 1)));
   XT_HIP(hipMalloc((void**) &c->d_qtotal, sizeof(long long) * xt::kQueues));
-  XT_HIP(hipMalloc((void**) &c->d_qctr, sizeof(unsigned long long) * xt::kQueues));
+  // the ticket counters and, after them, the run's error flag
+  XT_HIP(hipMalloc((void**) &c->d_qctr, sizeof(unsigned long long) * (xt::kQueues + 1)));
   XT_HIP(hipMalloc((void**) &c->d_acc, sizeof(unsigned long long) * 4 * nt));
   XT_HIP(hipMalloc((void**) &c->d_cost, sizeof(unsigned long long) * nt));
   XT_HIP(hipMalloc((void**) &c->d_task_e, sizeof(double) * 2 * nt));
@@ -921,6 +966,7 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
   R.qbeg    = c->d_qbeg;
   R.qtotal  = c->d_qtotal;
   R.qctr    = c->d_qctr;
+  R.err     = c->d_qctr + xt::kQueues;
   R.acc     = c->d_acc;
   R.cost    = o.record_task_time ? c->d_cost : nullptr;
   R.prog    = c->d_prog;
@@ -940,7 +986,7 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
   // the launch sequence; counters and accumulators are cleared by fill
   // kernels (no memset nodes when captured)
   auto body = [&](bool events) -> int {
-    int err = xt::launch_zero_u64(c->d_qctr, xt::kQueues, st);
+    int err = xt::launch_zero_u64(c->d_qctr, xt::kQueues + 1, st);
     if(!err) err = xt::launch_zero_u64(c->d_acc, 4 * std::max<long>(nt, 1), st);
     if(!err) err = xt::launch_zero_u64(c->d_cost, std::max<long>(nt, 1), st);
     if(err) return fail(XT_EHIP, std::string("zero: ") + hipGetErrorString((hipError_t) err));
@@ -995,6 +1041,19 @@ static int task_times(xt_ctx* c, long nt, float* task_ms) {
   int rate_khz = 100000;
   (void) hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device);
   for(long k = 0; k < nt; k++) task_ms[k] = (float) ((double) ticks[k] / rate_khz);
+  return XT_OK;
+}
+
+// Error flag of the last run (after its stream has drained): a box whose
+// energy partial was not finite or beyond 2^62 Eh did not enter the exact
+// fixed-point sums, so the run's energies are not returned.
+static int check_run_flag(xt_ctx* c, const char* fn) {
+  unsigned long long flag = 0;
+  XT_HIP(hipMemcpy(&flag, c->d_qctr + xt::kQueues, sizeof(flag), hipMemcpyDeviceToHost));
+  if(flag)
+    return fail(XT_ERANGE, std::string(fn) +
+                             ": an energy partial was not finite or beyond 2^62 Eh (zero or "
+                             "non-finite orbital-energy denominators, or non-finite inputs)");
   return XT_OK;
 }
 
@@ -1073,6 +1132,11 @@ int xt_run(xt_ctx* c, const xt_opts* opts, double* e45, double* task_partials, f
   float ms = 0;
   XT_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
+  if((rc = check_run_flag(c, "xt_run"))) {
+    e45[0] = e45[1] = std::nan("");
+    if(task_partials) std::fill(task_partials, task_partials + 2 * nt, std::nan(""));
+    return rc;
+  }
   if(o.record_task_time && task_ms) return task_times(c, nt, task_ms);
   return XT_OK;
 }
@@ -1131,6 +1195,11 @@ int xt_run_tasks(xt_ctx* c, const xt_opts* opts, long n, const long* which, doub
   float ms = 0;
   XT_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
+  if((rc = check_run_flag(c, "xt_run_tasks"))) {
+    e45[0] = e45[1] = std::nan("");
+    if(task_partials) std::fill(task_partials, task_partials + 2 * n, std::nan(""));
+    return rc;
+  }
   e45[0] = e45[1] = 0.0;
   for(long i = 0; i < n; i++) {  // listed order
     e45[0] += te[2 * which[i]];

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "xt_device.h"
+#include "xt_fixed.h"
 
 namespace xt {
 
@@ -342,23 +343,18 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, uint64_t d1code, 
 // scale 2^64 (acc[0] low word, acc[1] high word).  Integer addition is
 // associative, so the final value does not depend on the order of the adds:
 // every task's energy is bit-identical under any schedule.  |p| < 2^53.
-__device__ __forceinline__ void fx_add(unsigned long long* acc, double p) {
-  const bool         neg  = p < 0.0;
-  const double       a    = fabs(p);
-  unsigned long long hi   = (unsigned long long) a;  // floor
-  const double       frac = a - (double) hi;         // exact
-  unsigned long long lo   = (unsigned long long) (frac * 18446744073709551616.0);
-  if(neg) {
-    lo = ~lo + 1ull;
-    hi = ~hi + (lo == 0ull ? 1ull : 0ull);
+// Adds p into the exact fixed-point pair acc (xt_fixed.h); a partial that is
+// not finite or beyond 2^62 Eh sets *err instead, and the host reports the
+// run as failed.
+__device__ __forceinline__ void fx_add(unsigned long long* acc, double p, unsigned long long* err) {
+  unsigned long long lo, hi;
+  if(!fx_split(p, lo, hi)) {
+    atomicOr(err, 1ull);
+    return;
   }
   const unsigned long long old   = atomicAdd(&acc[0], lo);
   const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
   atomicAdd(&acc[1], hi + carry);
-}
-
-__device__ __forceinline__ double fx_value(const unsigned long long* acc) {
-  return (double) (long long) acc[1] + (double) acc[0] * 5.421010862427522170037264e-20;  // 2^-64
 }
 
 // Persistent kernel: workgroup b serves queue b % kQueues first (dispatch is
@@ -424,8 +420,8 @@ __global__ void __launch_bounds__(NT) k_spatial_fp64(SpatialDev S, RunDev R) {
       double                   e4, e5;
       spatial_subtile(sm, S, T.d1code, T.d2code, T.s1code, e4, e5);
       if(tid == 0) {
-        fx_add(&R.acc[4 * (size_t) ti], e4);
-        fx_add(&R.acc[4 * (size_t) ti + 2], e5);
+        fx_add(&R.acc[4 * (size_t) ti], e4, R.err);
+        fx_add(&R.acc[4 * (size_t) ti + 2], e5, R.err);
         if(R.cost) atomicAdd(&R.cost[ti], wall_clock64() - t0);
       }
     }

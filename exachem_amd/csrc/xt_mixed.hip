@@ -38,6 +38,7 @@
 #include <hip/hip_runtime.h>
 
 #include "xt_device.h"
+#include "xt_fixed.h"
 
 namespace xt {
 namespace {
@@ -95,15 +96,14 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-__device__ __forceinline__ void fx_add(unsigned long long* acc, double p) {
-  const bool         neg  = p < 0.0;
-  const double       a    = fabs(p);
-  unsigned long long hi   = (unsigned long long) a;
-  const double       frac = a - (double) hi;
-  unsigned long long lo   = (unsigned long long) (frac * 18446744073709551616.0);
-  if(neg) {
-    lo = ~lo + 1ull;
-    hi = ~hi + (lo == 0ull ? 1ull : 0ull);
+// Adds p into the exact fixed-point pair acc (xt_fixed.h); a partial that is
+// not finite or beyond 2^62 Eh sets *err instead, and the host reports the
+// run as failed.
+__device__ __forceinline__ void fx_add(unsigned long long* acc, double p, unsigned long long* err) {
+  unsigned long long lo, hi;
+  if(!fx_split(p, lo, hi)) {
+    atomicOr(err, 1ull);
+    return;
   }
   const unsigned long long old   = atomicAdd(&acc[0], lo);
   const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
@@ -691,8 +691,8 @@ __global__ void __launch_bounds__(NT) k_spatial_mixed(MixedDev M, RunDev R) {
       double e4, e5;
       box_energy(sm, M, T.s1code, e4, e5);
       if(tid == 0) {
-        fx_add(&R.acc[4 * (size_t) ti], e4);
-        fx_add(&R.acc[4 * (size_t) ti + 2], e5);
+        fx_add(&R.acc[4 * (size_t) ti], e4, R.err);
+        fx_add(&R.acc[4 * (size_t) ti + 2], e5, R.err);
         if(R.cost) atomicAdd(&R.cost[ti], wall_clock64() - t0);
       }
     }

@@ -40,10 +40,15 @@ extern "C" {
 #define XT_EHIP (-3)      /* HIP runtime error                                        */
 #define XT_ESTATE (-4)    /* call out of order (e.g. run before upload)               */
 #define XT_ENODEV (-5)    /* no usable gfx950 device                                  */
+#define XT_ERANGE (-6)    /* a non-finite input, or an energy partial that is not finite
+                             or beyond 2^62 Eh (e.g. a zero orbital-energy denominator):
+                             the run's energies are not returned (set to NaN)          */
 
 /* Precision of the contractions. */
 #define XT_PREC_FP64 0   /* FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 everywhere            */
-#define XT_PREC_MIXED 1  /* split BF16x3 products on MFMA, FP32 accumulation, FP64 energy */
+#define XT_PREC_MIXED 1  /* FP16 hi/lo split products (hi*hi + hi*lo + lo*hi) on FP16 MFMA
+                            with per-tile power-of-two scaling, FP32 accumulation, FP64
+                            denominators, energy partials and sums                         */
 
 /* Task ordering of the device work queue. */
 #define XT_SCHED_DRIVER 0   /* reference driver order (ccsd_t_fused_driver.hpp:368-383)   */
@@ -126,7 +131,8 @@ int xt_flops(xt_ctx* ctx, double* per_task, double* total, double* parts);
  *   ovoo[O][V][O][O]            (ia|jk)  (symmetric in j,k)
  *   vvvo[V][V][V][O]            (ab|ci)  (symmetric in a,b)
  * Copied to HBM and re-laid out there for the kernels.  Must follow
- * xt_set_tiling with the same O, V.
+ * xt_set_tiling with the same O, V.  Every element must be finite: the
+ * upload checks that before any device work and fails with XT_ERANGE.
  */
 /* Same inputs with the V2 blocks built on the device from Cholesky vectors,
  * in place of V2TensorSetup::setupV2Tensors (cholesky/v2tensors.cpp:52-98):
@@ -180,6 +186,8 @@ int xt_prepare(xt_ctx* ctx, int precision);
  * factors applied per task.  task_partials[2*n] (optional) receives each
  * task's (E[4], E[5]) contribution, factor included; task_ms[n] (optional)
  * receives per-task device time in ms when opts->record_task_time is set.
+ * XT_ERANGE if an energy partial was not finite or beyond 2^62 Eh (it cannot
+ * enter the exact fixed-point sums); e45 and task_partials are then NaN.
  */
 int xt_run(xt_ctx* ctx, const xt_opts* opts, double* e45, double* task_partials, float* task_ms);
 

@@ -45,15 +45,85 @@ namespace {
 using Index = size_t;
 using Tup6  = std::tuple<Index, Index, Index, Index, Index, Index>;
 
+// Counter-based generator of the synthetic closed-shell inputs, restated from
+// the product's host generator so that inputs too large for host memory
+// (O=64, V=512: vvvo alone is 68.7 GB) can be formed element by element.
+// The two are compared array for array on small sizes (tests/test_oracle.py),
+// and the GPU tests that upload the product's arrays reproduce energies
+// computed here from this one.
+inline uint64_t gen_mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+inline uint64_t gen_hash3(uint64_t seed, uint64_t stream, uint64_t idx) {
+  return gen_mix64(gen_mix64(seed ^ (stream * 0xD6E8FEB86659FD93ULL)) ^ gen_mix64(idx));
+}
+inline double gen_u01(uint64_t h) {
+  return ((double) (h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+inline double gen_gauss(uint64_t seed, uint64_t stream, uint64_t idx) {
+  const double u1 = gen_u01(gen_hash3(seed, stream, 2 * idx));
+  const double u2 = gen_u01(gen_hash3(seed, stream, 2 * idx + 1));
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586476925286766559 * u2);
+}
+enum GenStream : uint64_t { G_EO = 1, G_EV = 2, G_VVOO = 3, G_OVOO = 4, G_VVVO = 5, G_T1 = 6 };
+
+struct GenParams {
+  long                O = 0, V = 0;
+  uint64_t            seed  = 0;
+  double              sigma = 0.0, t1s = 0.0;
+  std::vector<double> eo, ev;  // sorted orbital energies
+  GenParams(long O_, long V_, uint64_t seed_, double sigma_, double t1s_)
+    : O(O_), V(V_), seed(seed_), sigma(sigma_), t1s(t1s_), eo(O_), ev(V_) {
+    for(long i = 0; i < O; i++) eo[i] = -2.0 + 1.7 * gen_u01(gen_hash3(seed, G_EO, i));
+    for(long a = 0; a < V; a++) ev[a] = 0.1 + 2.9 * gen_u01(gen_hash3(seed, G_EV, a));
+    std::sort(eo.begin(), eo.end());
+    std::sort(ev.begin(), ev.end());
+  }
+  double t1(long a, long i) const { return t1s * gen_gauss(seed, G_T1, a * O + i); }
+  // (ai|bj), drawn at the lexicographically smaller of (a,i,b,j), (b,j,a,i)
+  double vvoo(long a, long b, long i, long j) const {
+    const bool sw = (b < a) || (b == a && j < i);
+    const long a_ = sw ? b : a, i_ = sw ? j : i, b_ = sw ? a : b, j_ = sw ? i : j;
+    return sigma * gen_gauss(seed, G_VVOO, ((a_ * O + i_) * V + b_) * O + j_);
+  }
+  double t2(long a, long b, long i, long j) const {
+    return vvoo(a, b, i, j) / ((eo[i] + eo[j]) - (ev[a] + ev[b]));
+  }
+  double ovoo(long i, long a, long j, long k) const {  // (ia|jk), symmetric in j,k
+    return sigma * gen_gauss(seed, G_OVOO, ((i * V + a) * O + std::min(j, k)) * O + std::max(j, k));
+  }
+  double vvvo(long a, long b, long c, long i) const {  // (ab|ci), symmetric in a,b
+    return sigma * gen_gauss(seed, G_VVVO, ((std::min(a, b) * V + std::max(a, b)) * V + c) * O + i);
+  }
+};
+
 // Closed-shell spatial inputs from which spin-orbital elements are formed
-// (oracle_expand_closed_shell documents the formulas).
+// (oracle_expand_closed_shell documents the formulas): arrays, or the
+// generator when gen is set.
 struct SpatialSrc {
-  long          O = 0, V = 0;
-  const double* t1s  = nullptr;  // [V][O]
-  const double* t2s  = nullptr;  // [V][V][O][O]
-  const double* vvoo = nullptr;  // [V][V][O][O] (ai|bj)
-  const double* ovoo = nullptr;  // [O][V][O][O] (ia|jk)
-  const double* vvvo = nullptr;  // [V][V][V][O] (ab|ci)
+  long             O = 0, V = 0;
+  const double*    t1s  = nullptr;  // [V][O]
+  const double*    t2s  = nullptr;  // [V][V][O][O]
+  const double*    vvoo = nullptr;  // [V][V][O][O] (ai|bj)
+  const double*    ovoo = nullptr;  // [O][V][O][O] (ia|jk)
+  const double*    vvvo = nullptr;  // [V][V][V][O] (ab|ci)
+  const GenParams* gen  = nullptr;
+  double T1(long a, long i) const { return gen ? gen->t1(a, i) : t1s[a * O + i]; }
+  double T2(long a, long b, long i, long j) const {
+    return gen ? gen->t2(a, b, i, j) : t2s[((a * V + b) * O + i) * O + j];
+  }
+  double VVOO(long a, long b, long i, long j) const {
+    return gen ? gen->vvoo(a, b, i, j) : vvoo[((a * V + b) * O + i) * O + j];
+  }
+  double OVOO(long i, long a, long j, long k) const {
+    return gen ? gen->ovoo(i, a, j, k) : ovoo[((i * V + a) * O + j) * O + k];
+  }
+  double VVVO(long a, long b, long c, long i) const {
+    return gen ? gen->vvvo(a, b, c, i) : vvvo[((a * V + b) * V + c) * O + i];
+  }
 };
 
 enum Kind { K_T1 = 0, K_T2 = 1, K_IJAB = 2, K_IJKA = 3, K_IABC = 4 };
@@ -69,35 +139,31 @@ inline double so_elem(const SpatialSrc& S, int kind, const long* q) {
   switch(kind) {
     case K_T1: {  // t1(a,i)
       long sa, si, A = vir(q[0], &sa), I = occ(q[1], &si);
-      return sa == si ? S.t1s[A * O + I] : 0.0;
+      return sa == si ? S.T1(A, I) : 0.0;
     }
     case K_T2: {  // t2(a,b,i,j) = [sa=si][sb=sj] t_ij^ab - [sa=sj][sb=si] t_ji^ab
       long sa, sb, si, sj, A = vir(q[0], &sa), B = vir(q[1], &sb), I = occ(q[2], &si),
                            J = occ(q[3], &sj);
       bool c1 = sa == si && sb == sj, c2 = sa == sj && sb == si;
-      return comb(c1, c2, c1 ? S.t2s[((A * V + B) * O + I) * O + J] : 0.0,
-                  c2 ? S.t2s[((A * V + B) * O + J) * O + I] : 0.0);
+      return comb(c1, c2, c1 ? S.T2(A, B, I, J) : 0.0, c2 ? S.T2(A, B, J, I) : 0.0);
     }
     case K_IJAB: {  // <ij||ab> = (ia|jb) - (ib|ja), (ia|jb) = (ai|bj)
       long si, sj, sa, sb, I = occ(q[0], &si), J = occ(q[1], &sj), A = vir(q[2], &sa),
                            B = vir(q[3], &sb);
       bool c1 = si == sa && sj == sb, c2 = si == sb && sj == sa;
-      return comb(c1, c2, c1 ? S.vvoo[((A * V + B) * O + I) * O + J] : 0.0,
-                  c2 ? S.vvoo[((B * V + A) * O + I) * O + J] : 0.0);
+      return comb(c1, c2, c1 ? S.VVOO(A, B, I, J) : 0.0, c2 ? S.VVOO(B, A, I, J) : 0.0);
     }
     case K_IJKA: {  // <ij||ka> = (ik|ja) - (jk|ia), (ik|ja) = (ja|ik)
       long si, sj, sk, sa, I = occ(q[0], &si), J = occ(q[1], &sj), K = occ(q[2], &sk),
                            A = vir(q[3], &sa);
       bool c1 = si == sk && sj == sa, c2 = sj == sk && si == sa;
-      return comb(c1, c2, c1 ? S.ovoo[((J * V + A) * O + I) * O + K] : 0.0,
-                  c2 ? S.ovoo[((I * V + A) * O + J) * O + K] : 0.0);
+      return comb(c1, c2, c1 ? S.OVOO(J, A, I, K) : 0.0, c2 ? S.OVOO(I, A, J, K) : 0.0);
     }
     default: {  // K_IABC: <ia||bc> = (ib|ac) - (ic|ab), (ib|ac) = (ac|bi)
       long si, sa, sb, sc, I = occ(q[0], &si), A = vir(q[1], &sa), B = vir(q[2], &sb),
                            C = vir(q[3], &sc);
       bool c1 = si == sb && sa == sc, c2 = si == sc && sa == sb;
-      return comb(c1, c2, c1 ? S.vvvo[((A * V + C) * V + B) * O + I] : 0.0,
-                  c2 ? S.vvvo[((A * V + B) * V + C) * O + I] : 0.0);
+      return comb(c1, c2, c1 ? S.VVVO(A, C, B, I) : 0.0, c2 ? S.VVVO(A, B, C, I) : 0.0);
     }
   }
 }
@@ -1037,6 +1103,63 @@ int oracle_run_spatial(int noab, int nvab, const int* tiles, const int* spins, c
   S.vvvo = vvvo;
   SOProblem P = make_problem(noab, nvab, tiles, spins, evl, nullptr, nullptr, nullptr, nullptr,
                              nullptr);
+  if(P.nO != 2 * (size_t) O || P.nV != 2 * (size_t) V) return -2;
+  P.src = &S;
+  return run_impl(P, is_restricted, nwhich, which, e_out, per_task);
+}
+
+// Synthetic inputs of the counter-based generator as arrays (any output may
+// be null): the same layouts and values as the product's xt_synth_closed_shell.
+void oracle_gen_synth(int O, int V, uint64_t seed, double sigma, double t1_sigma, double* eo,
+                      double* ev, double* t1, double* t2, double* vvoo, double* ovoo,
+                      double* vvvo) {
+  const GenParams G(O, V, seed, sigma, t1_sigma);
+  const long      o = O, v = V;
+  if(eo) std::copy(G.eo.begin(), G.eo.end(), eo);
+  if(ev) std::copy(G.ev.begin(), G.ev.end(), ev);
+  if(t1)
+    for(long a = 0; a < v; a++)
+      for(long i = 0; i < o; i++) t1[a * o + i] = G.t1(a, i);
+#pragma omp parallel for
+  for(long a = 0; a < v; a++)
+    for(long b = 0; b < v; b++)
+      for(long i = 0; i < o; i++)
+        for(long j = 0; j < o; j++) {
+          const size_t id = ((a * v + b) * o + i) * o + j;
+          if(vvoo) vvoo[id] = G.vvoo(a, b, i, j);
+          if(t2) t2[id] = G.t2(a, b, i, j);
+        }
+  if(ovoo)
+#pragma omp parallel for
+    for(long i = 0; i < o; i++)
+      for(long a = 0; a < v; a++)
+        for(long j = 0; j < o; j++)
+          for(long k = 0; k < o; k++) ovoo[((i * v + a) * o + j) * o + k] = G.ovoo(i, a, j, k);
+  if(vvvo)
+#pragma omp parallel for
+    for(long a = 0; a < v; a++)
+      for(long b = 0; b < v; b++)
+        for(long c = 0; c < v; c++)
+          for(long i = 0; i < o; i++) vvvo[((a * v + b) * v + c) * o + i] = G.vvvo(a, b, c, i);
+}
+
+// oracle_run_spatial on the generator's inputs, each element formed where the
+// gathers read it, so no input array is held (orbital energies in tile order
+// |occ_a|occ_b|virt_a|virt_b| from the generator's eo, ev).
+int oracle_run_generated(int noab, int nvab, const int* tiles, const int* spins,
+                         int is_restricted, int O, int V, uint64_t seed, double sigma,
+                         double t1_sigma, long nwhich, const long* which, double* e_out,
+                         double* per_task) {
+  const GenParams     G(O, V, seed, sigma, t1_sigma);
+  std::vector<double> evl;
+  for(int s = 0; s < 2; s++) evl.insert(evl.end(), G.eo.begin(), G.eo.end());
+  for(int s = 0; s < 2; s++) evl.insert(evl.end(), G.ev.begin(), G.ev.end());
+  SpatialSrc S;
+  S.O   = O;
+  S.V   = V;
+  S.gen = &G;
+  SOProblem P = make_problem(noab, nvab, tiles, spins, evl.data(), nullptr, nullptr, nullptr,
+                             nullptr, nullptr);
   if(P.nO != 2 * (size_t) O || P.nV != 2 * (size_t) V) return -2;
   P.src = &S;
   return run_impl(P, is_restricted, nwhich, which, e_out, per_task);

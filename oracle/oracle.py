@@ -28,6 +28,11 @@ What each function restates and how it is pinned:
   other task does not (tests/test_oracle.py).  Energy sums are carried
   in long double (the reference uses double), so the values are the correctly
   rounded sums of the per-element terms.
+* ``gen_synth`` / ``run_generated`` -- the product's counter-based input
+  generator restated, so inputs too large for host memory (O=64, V=512) are
+  formed element by element where the gathers read them.  Pinned bit for bit
+  against the product's ``synth_closed_shell`` arrays, and ``run_generated``
+  against ``run_spatial`` on those arrays (tests/test_oracle.py).
 * ``task_packed`` -- the packed buffers handed to the GPU launcher
   (ccsd_t_all_fused.hpp:144-213).  Pinned through ``run`` (same code path).
 """
@@ -84,6 +89,13 @@ def lib():
         L.oracle_task_packed.argtypes = [C.c_int, C.c_int, ip, ip, C.c_int] + [dp] * 5 + [
             lp] + [dp] * 6 + [ip, ip]
         L.oracle_num_threads.restype = C.c_int
+        L.oracle_gen_synth.restype = None
+        L.oracle_gen_synth.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_double,
+                                       C.c_double] + [dp] * 7
+        L.oracle_run_generated.restype = C.c_int
+        L.oracle_run_generated.argtypes = [C.c_int, C.c_int, ip, ip, C.c_int, C.c_int, C.c_int,
+                                           C.c_uint64, C.c_double, C.c_double, C.c_long, lp,
+                                           dp, dp]
         _lib = L
     return _lib
 
@@ -232,6 +244,43 @@ def run_spatial(t: Tiling, evl, s, restricted=True, which=None):
                               *[p for _, p in arrs], nw, whp, ep, ptp)
     if rc != 0:
         raise RuntimeError(f"oracle_run_spatial failed ({rc})")
+    return e[0], e[1], pt[:2 * n].reshape(n, 2)
+
+
+def gen_synth(O, V, seed, sigma=0.05, t1_sigma=1e-3):
+    """The synthetic closed-shell inputs of the counter-based generator
+    (restated from the product's), as a dict of arrays like the product's
+    synth_closed_shell returns."""
+    s = {"eo": np.zeros(O), "ev": np.zeros(V), "t1": np.zeros((V, O)),
+         "t2": np.zeros((V, V, O, O)), "vvoo": np.zeros((V, V, O, O)),
+         "ovoo": np.zeros((O, V, O, O)), "vvvo": np.zeros((V, V, V, O))}
+    lib().oracle_gen_synth(O, V, seed, sigma, t1_sigma,
+                           *[s[k].ctypes.data_as(C.POINTER(C.c_double))
+                             for k in ("eo", "ev", "t1", "t2", "vvoo", "ovoo", "vvvo")])
+    return s
+
+
+def run_generated(t: Tiling, O, V, seed, sigma=0.05, t1_sigma=1e-3, restricted=True,
+                  which=None):
+    """``run_spatial`` on the generator's inputs with every element formed
+    where it is read, so no input array is held (O=64, V=512 fits).
+    Returns (E4, E5, per_task[n,2])."""
+    L = lib()
+    tiles, tp = _i(t.tiles)
+    spins, sp = _i(t.spins)
+    if which is None:
+        n = L.oracle_num_tasks(t.noab, t.nvab, tp, sp, int(restricted))
+        wh, whp = _l(np.zeros(1, dtype=np.int64))
+        nw = 0
+    else:
+        wh, whp = _l(np.asarray(which, dtype=np.int64))
+        n = nw = len(wh)
+    e, ep = _d(np.zeros(2))
+    pt, ptp = _d(np.zeros(2 * max(n, 1)))
+    rc = L.oracle_run_generated(t.noab, t.nvab, tp, sp, int(restricted), O, V, seed, sigma,
+                                t1_sigma, nw, whp, ep, ptp)
+    if rc != 0:
+        raise RuntimeError(f"oracle_run_generated failed ({rc})")
     return e[0], e[1], pt[:2 * n].reshape(n, 2)
 
 

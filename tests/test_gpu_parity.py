@@ -328,3 +328,32 @@ def test_memory_required_matches_allocations(xt):
     assert abs(used_split - req["split"]) <= 0.02 * req["split"] + 64 * 2**20
     assert req["operands"] + req["staging"] < total
     ctx.close()
+
+
+def test_non_finite_partial_is_reported_not_summed(xt):
+    """Orbital energies that make every denominator zero (all occupied and
+    virtual energies equal) give infinite box partials.  They cannot enter the
+    exact fixed-point sums, so the run fails with XT_ERANGE and NaN energies
+    instead of a finite, wrong E[4]/E[5]; the next run on the same context,
+    with valid energies, is unaffected (the flag is cleared per run)."""
+    case = next(c for c in small_cases() if c["name"] == "uneven_small")
+    s = xt.synth_closed_shell(case["O"], case["V"], case["seed"], case["sigma"],
+                              case["t1_sigma"])
+    tiling = xt.Tiling(case["O"], case["V"], case["occ_tiles"], case["vir_tiles"])
+    bad = dict(s, eo=np.ones_like(s["eo"]), ev=np.ones_like(s["ev"]))
+    ctx = xt.CCSDTriples(0)
+    ctx.set_tiling(tiling)
+    for prec in (xt.PREC_FP64, xt.PREC_MIXED):
+        print(f"[gpu] zero denominators, precision {prec}", flush=True)
+        ctx.upload(bad)
+        with pytest.raises(xt.XtError) as e:
+            ctx.run(precision=prec, use_graph=1)
+        assert e.value.code == xt.XT_ERANGE, str(e.value)
+        with pytest.raises(xt.XtError) as e:
+            ctx.run_tasks([0, 1], precision=prec)
+        assert e.value.code == xt.XT_ERANGE
+        ctx.upload(s)
+        r = ctx.run(precision=prec, use_graph=1)
+        tol = (1e-10 + 2e-14 * abs(case["E4"])) if prec == xt.PREC_FP64 else 1e-7 * abs(case["E4"])
+        assert abs(r["e4"] - case["E4"]) <= tol and abs(r["e5"] - case["E5"]) <= tol
+    ctx.close()
