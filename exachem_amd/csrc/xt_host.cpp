@@ -20,6 +20,7 @@
 #include "../../include/exachem_t.h"
 #include "xt_device.h"
 #include "xt_internal.h"
+#include "xt_queue.h"
 
 namespace {
 thread_local std::string g_err;
@@ -842,12 +843,7 @@ struct Queues {
 // Tickets of task T under group shape glog (one group of 64 boxes per
 // ceil(n/2^glog) cell of the task's box grid).
 static long long task_tickets(const xt::TaskDev& T, const int* glog, bool mixed) {
-  long long n = 1ll << xt::kGroupLog2;
-  for(int p = 0; p < 6; p++) {
-    const int b = mixed ? T.nsubm[p] : T.nsub[p];
-    n *= (b + (1 << glog[p]) - 1) >> glog[p];
-  }
-  return n;
+  return xt::task_ticket_count(mixed ? T.nsubm : T.nsub, glog);
 }
 
 static Queues build_queues(const Plan& P, const std::vector<int>& sel, int schedule,
@@ -1214,6 +1210,47 @@ int xt_run_tasks(xt_ctx* c, const xt_opts* opts, long n, const long* which, doub
     if((rc = task_times(c, nt, all.data()))) return rc;
     for(long i = 0; i < n; i++) task_ms[i] = all[which[i]];
   }
+  return XT_OK;
+}
+
+int xt_queue_layout(xt_ctx* c, const xt_opts* opts, long n, const long* which, int* qbeg,
+                    int* qtask, long long* qtick, long long* qtotal, int* nbox, int* glog) {
+  if(!c || !opts || n < 0 || (n > 0 && !which) || !qbeg || !qtask || !qtick || !qtotal || !nbox ||
+     !glog)
+    return fail(XT_EINVAL, "xt_queue_layout: bad argument");
+  if(!c->have_plan) return fail(XT_ESTATE, "xt_queue_layout: no tiling");
+  if(opts->precision != XT_PREC_FP64 && opts->precision != XT_PREC_MIXED)
+    return fail(XT_EINVAL, "xt_queue_layout: unknown precision");
+  const long       nt = (long) c->plan.dev.size();
+  std::vector<int> sel;
+  if(n == 0) {
+    sel.resize(nt);
+    std::iota(sel.begin(), sel.end(), 0);
+  }
+  else {
+    std::vector<char> seen(nt, 0);
+    for(long i = 0; i < n; i++) {
+      if(which[i] < 0 || which[i] >= nt) return fail(XT_EINVAL, "xt_queue_layout: task out of range");
+      if(seen[which[i]]) return fail(XT_EINVAL, "xt_queue_layout: repeated task");
+      seen[which[i]] = 1;
+      sel.push_back((int) which[i]);
+    }
+  }
+  // the same choice of costs and group shape as run_units
+  const std::vector<double>* learned =
+    (opts->schedule == XT_SCHED_LEARNED && (long) c->learned_cost.size() == nt) ? &c->learned_cost
+                                                                               : nullptr;
+  const bool mixed = opts->precision == XT_PREC_MIXED;
+  const int* g     = mixed ? c->glog_mixed : c->glog_fp64;
+  Queues     Q     = build_queues(c->plan, sel, opts->schedule, learned, mixed, g);
+  std::copy(Q.beg.begin(), Q.beg.end(), qbeg);
+  std::copy(Q.task.begin(), Q.task.end(), qtask);
+  std::copy(Q.tick.begin(), Q.tick.end(), qtick);
+  std::copy(Q.total.begin(), Q.total.end(), qtotal);
+  for(long t = 0; t < nt; t++)
+    for(int p = 0; p < 6; p++)
+      nbox[6 * t + p] = mixed ? c->plan.dev[t].nsubm[p] : c->plan.dev[t].nsub[p];
+  std::copy(g, g + 6, glog);
   return XT_OK;
 }
 

@@ -39,6 +39,7 @@
 
 #include "xt_device.h"
 #include "xt_fixed.h"
+#include "xt_queue.h"
 
 namespace xt {
 namespace {
@@ -635,14 +636,9 @@ __global__ void __launch_bounds__(NT) k_spatial_mixed(MixedDev M, RunDev R) {
         if(R.prog) __hip_atomic_store(&R.prog[q], t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         if((long long) t >= R.qtotal[q]) s_task = -1;
         else {
-          int lo = R.qbeg[q], hi = R.qbeg[q + 1] - 1;
-          while(lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if(R.qtick[mid] <= (long long) t) lo = mid;
-            else hi = mid - 1;
-          }
-          s_task  = R.qtask[lo];
-          s_local = (long long) t - R.qtick[lo];
+          const int e = queue_entry(R.qbeg, R.qtick, q, (long long) t);
+          s_task      = R.qtask[e];
+          s_local     = (long long) t - R.qtick[e];
         }
       }
       __syncthreads();
@@ -650,21 +646,8 @@ __global__ void __launch_bounds__(NT) k_spatial_mixed(MixedDev M, RunDev R) {
       if(ti < 0) break;
       const TaskDev& T   = R.tasks[ti];
       // ticket -> box: the low bits select the box inside its group
-      long long box = s_local >> kGroupLog2;
-      const int inb = (int) (s_local & ((1 << kGroupLog2) - 1));
-      bool      ok  = true;
-      int       sub[6];
-      int       shift = 0;
-#pragma unroll
-      for(int p = 0; p < 6; p++) {
-        const int gl = R.glog[p];
-        const int nb = (T.nsubm[p] + (1 << gl) - 1) >> gl;
-        sub[p]       = ((int) (box % nb) << gl) + ((inb >> shift) & ((1 << gl) - 1));
-        box /= nb;
-        shift += gl;
-        ok = ok && sub[p] < T.nsubm[p];
-      }
-      if(!ok) continue;
+      int sub[6];
+      if(!ticket_box(s_local, T.nsubm, R.glog, sub)) continue;
 #pragma unroll
       for(int p = 0; p < 6; p++)
         if(tid == p) {

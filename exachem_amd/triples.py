@@ -342,6 +342,29 @@ class CCSDTriples:
         names = ("s1_t1", "s1_v2", "d1_t2", "d1_v2", "d2_t2", "d2_v2", "max_d1", "max_d2")
         return {n: int(v) for n, v in zip(names, out)}
 
+    def queue_layout(self, which=None, precision=PREC_FP64, schedule=SCHED_DRIVER):
+        """The work queues a run over `which` (all tasks if None) would launch
+        with: dict of qbeg[9], qtask, qtick, qtotal[8], nbox[ntasks][6] and
+        the group shape glog[6].  Host only."""
+        nt = self.num_tasks()
+        w = (np.zeros(0, dtype=np.int64) if which is None
+             else np.ascontiguousarray(which, dtype=np.int64))
+        ne = nt if which is None else len(w)
+        o = default_opts(precision=precision, schedule=schedule)
+        qbeg = np.zeros(9, dtype=np.int32)
+        qtask = np.zeros(max(ne, 1), dtype=np.int32)
+        qtick = np.zeros(max(ne, 1), dtype=np.int64)
+        qtotal = np.zeros(8, dtype=np.int64)
+        nbox = np.zeros(6 * max(nt, 1), dtype=np.int32)
+        glog = np.zeros(6, dtype=np.int32)
+        ll = C.POINTER(C.c_longlong)
+        check(lib().xt_queue_layout(self._ctx, C.byref(o), len(w),
+                                    w.ctypes.data_as(C.POINTER(C.c_long)), _ip(qbeg), _ip(qtask),
+                                    qtick.ctypes.data_as(ll), qtotal.ctypes.data_as(ll),
+                                    _ip(nbox), _ip(glog)))
+        return dict(qbeg=qbeg, qtask=qtask[:ne], qtick=qtick[:ne], qtotal=qtotal,
+                    nbox=nbox[:6 * nt].reshape(nt, 6), glog=glog)
+
     def gather_task(self, tiles, bufs, stream=None):
         """Device-side gather of one task's packed buffers, in place of
         ccsd_t_data_{s1,d1,d2}_new (ccsd_t_all_fused.hpp:149-178).

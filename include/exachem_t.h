@@ -224,6 +224,17 @@ int xt_sum_task_energies(long n, const double* task_partials, double* e45);
  * b1, b2). */
 int xt_task_codes(xt_ctx* ctx, long task, uint64_t* codes);
 
+/* Work-queue layout a run over `which` (n tasks; n = 0: every task) would
+ * launch with under opts' precision and schedule (introspection for tests
+ * and tools; host only, no device work).  Eight queues, one per XCD: queue q
+ * holds the entries qbeg[q]..qbeg[q+1]-1 (qbeg[9]); entry e is task qtask[e]
+ * (n or ntasks entries) owning the queue's tickets from qtick[e]; qtotal[8]
+ * tickets per queue.  nbox[6*ntasks]: every task's box grid at that
+ * precision; glog[6]: the ticket group shape in use.  Replaces the NXTVAL
+ * task counter of ccsd_t_fused_driver.hpp:169-172,381,456. */
+int xt_queue_layout(xt_ctx* ctx, const xt_opts* opts, long n, const long* which, int* qbeg,
+                    int* qtask, long long* qtick, long long* qtotal, int* nbox, int* glog);
+
 /* ------------------------------------ device-side gather (per-task path) */
 /*
  * Replaces the host gathers ccsd_t_data_s1_new (ccsd_t_all_fused_singles.hpp:559-580),

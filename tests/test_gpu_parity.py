@@ -330,6 +330,102 @@ def test_memory_required_matches_allocations(xt):
     ctx.close()
 
 
+def _large_fixture(name):
+    path = os.path.join(HERE, "golden", name)
+    if not os.path.exists(path):
+        pytest.fail(f"fixture {name} missing: run tests/golden/make_golden.py")
+    return json.load(open(path))
+
+
+def _check_large_sample(xt, case, ctx, prec, schedule, label):
+    """Each sampled task against the oracle: FP64 within ABS + REL |E|,
+    mixed within MIX_TASK_REL |E4_task| + the Pauli floor; the sums of the
+    sampled tasks' energies too."""
+    which = case["which"]
+    ref = np.asarray(case["task_partials"])
+    r = ctx.run_tasks(which, precision=prec, schedule=schedule)
+    got = r["task_partials"]
+    floor = MIX_TASK_FLOOR * np.max(np.abs(ref[:, 0]))
+    for i, k in enumerate(which):
+        d = np.abs(got[i] - ref[i])
+        print(f"[gpu] {label} precision {prec} task {k} dims {case['task_dims'][i]}: "
+              f"|dE4| {d[0]:.2e} |dE5| {d[1]:.2e} (E4 {ref[i, 0]:.6e})", flush=True)
+        if prec == xt.PREC_FP64:
+            assert close(got[i, 0], ref[i, 0]) and close(got[i, 1], ref[i, 1]), (k, got[i], ref[i])
+        else:
+            tol = MIX_TASK_REL * abs(ref[i, 0]) + floor
+            assert d[0] <= tol and d[1] <= tol, (k, got[i], ref[i])
+    return got
+
+
+def _learned_costs(xt, ctx, prec):
+    """Trains the cost model on a small timed sample and installs its
+    predictions, so the samples run under SCHED_LEARNED."""
+    model = xt.CostModel(0, seed=1)
+    stats = ctx.learn_schedule(model, prec, n_sample=48, precisions=(prec,), steps=300)
+    model.close()
+    print(f"[gpu] learned schedule: {stats}", flush=True)
+
+
+@pytest.mark.parametrize("fixture", ["parity_c3_sample.json", "parity_c3_wide_sample.json"])
+def test_o48_v384_uneven_sampled_tasks(xt, fixture):
+    """O=48, V=384 with seeded uneven tiles (occupied U[6,20], virtual
+    U[16,48]), the full inputs resident: seeded tasks across the flop
+    quantiles plus the widest-tile, all-equal-tile (factor 1/6), two-equal
+    (1/2) and mixed-spin tasks, FP64 and mixed under the learned schedule,
+    against the oracle's energies for the same tasks.  The second fixture
+    swaps in a 20-wide occupied tile (the seeded tiling's widest is 15)."""
+    case = _large_fixture(fixture)
+    print(f"[gpu] {case['name']}: tasks {case['which']} roles {case['roles']}", flush=True)
+    ctx = xt.CCSDTriples(0)
+    ctx.set_tiling(xt.Tiling(case["O"], case["V"], case["occ_tiles"], case["vir_tiles"]))
+    assert ctx.num_tasks() == case["n_tasks"]
+    ctx.synth_upload(case["seed"], case["sigma"], case["t1_sigma"])
+    for prec in (xt.PREC_FP64, xt.PREC_MIXED):
+        _learned_costs(xt, ctx, prec)
+        _check_large_sample(xt, case, ctx, prec, xt.SCHED_LEARNED, case["name"])
+    ctx.close()
+
+
+def test_o48_v384_physical_scale_sampled_tasks(xt):
+    """The O=48, V=384 sample's tasks with the integrals scaled by 2^-8, so
+    that E(T) of the whole workload is ~0.2 Eh: the mixed form's absolute
+    error on each task and on their sum stays far below 1 uEh, against the
+    oracle (not against the GPU FP64 form)."""
+    case = _large_fixture("parity_c3_phys_sample.json")
+    ctx = xt.CCSDTriples(0)
+    ctx.set_tiling(xt.Tiling(case["O"], case["V"], case["occ_tiles"], case["vir_tiles"]))
+    ctx.synth_upload(case["seed"], case["sigma"], case["t1_sigma"])
+    ref = np.asarray(case["task_partials"])
+    for prec in (xt.PREC_FP64, xt.PREC_MIXED):
+        got = _check_large_sample(xt, case, ctx, prec, xt.SCHED_ANALYTIC, case["name"])
+        d = np.abs(got - ref)
+        ds = np.abs(got.sum(axis=0) - ref.sum(axis=0))
+        print(f"[gpu] physical scale precision {prec}: max task |dE4| {d[:, 0].max():.2e} "
+              f"|dE(T)| {d[:, 1].max():.2e}; sum |dE4| {ds[0]:.2e} |dE(T)| {ds[1]:.2e} "
+              f"(sum E(T) {ref[:, 1].sum():.6e})", flush=True)
+        assert d.max() <= 1e-6 and ds.max() <= 1e-6
+    ctx.close()
+
+
+def test_o64_v512_uneven_sampled_tasks(xt):
+    """O=64, V=512, seeded uneven tiles: the largest single-GPU workload,
+    full inputs resident (182.5 GB of FP64 operands, 87 GB of split
+    copies), sampled tasks in FP64 and mixed under the learned schedule
+    against the oracle (which formed only the input elements those tasks
+    read)."""
+    case = _large_fixture("parity_c4_sample.json")
+    print(f"[gpu] {case['name']}: tasks {case['which']} roles {case['roles']}", flush=True)
+    ctx = xt.CCSDTriples(0)
+    ctx.set_tiling(xt.Tiling(case["O"], case["V"], case["occ_tiles"], case["vir_tiles"]))
+    assert ctx.num_tasks() == case["n_tasks"]
+    ctx.synth_upload(case["seed"], case["sigma"], case["t1_sigma"])
+    for prec in (xt.PREC_FP64, xt.PREC_MIXED):
+        _learned_costs(xt, ctx, prec)
+        _check_large_sample(xt, case, ctx, prec, xt.SCHED_LEARNED, case["name"])
+    ctx.close()
+
+
 def test_non_finite_partial_is_reported_not_summed(xt):
     """Orbital energies that make every denominator zero (all occupied and
     virtual energies equal) give infinite box partials.  They cannot enter the
