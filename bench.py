@@ -30,15 +30,17 @@ kernel's HIP-event time.  Beside it:
     a physical energy scale (integrals scaled so that |E(T)| is below 1 Eh);
   * cpu_baseline: the CPU oracle (a C++ restatement of the reference's CPU
     fused (T) path) timed on a seeded, size-stratified task sample (two
-    tasks per flop quartile, smallest first, until its budget is spent),
-    extrapolated by the flop count; its energies on those tasks are checked
-    against the GPU too (cpu_sample_check).
+    tasks per flop quartile, smallest first) in a host thread that runs
+    beside the GPU work from the upload to the end of the timed steps, on
+    all host cores but one, extrapolated by the flop count; its energies on
+    those tasks are checked against the GPU too (cpu_sample_check).
 """
 import argparse
 import ctypes as C
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -290,43 +292,62 @@ def cholesky_leg(xt, tiling, kfac, cpu=True):
     return out
 
 
-def cpu_baseline(tiling, s, per_task, budget_s, n_per_quartile=2):
+class CpuBaseline(threading.Thread):
     """Times the oracle on a seeded, size-stratified sample of the workload's
-    tasks: n_per_quartile tasks drawn from each flop quartile, run smallest
-    first until the budget is spent (at least one task)."""
-    from oracle import oracle as O
-    O.build()
-    occ, vir = tiling.occ, tiling.vir
-    t = O.Tiling(occ, occ, vir, vir)
-    evl = np.concatenate([s["eo"], s["eo"], s["ev"], s["ev"]])
-    order = np.argsort(per_task, kind="stable")
-    rng = np.random.default_rng(SEED)
-    strata = np.array_split(order, 4)
-    sample = np.concatenate([rng.choice(q, min(n_per_quartile, len(q)), replace=False)
-                             for q in strata])
-    sample = sample[np.argsort(per_task[sample], kind="stable")]
-    pick, parts, secs = [], [], []
-    for k in sample:
-        if pick and sum(secs) > budget_s:
-            break
-        t0 = time.time()
-        _, _, pt = O.run_spatial(t, evl, s, True, [int(k)])
-        secs.append(time.time() - t0)
-        pick.append(int(k))
-        parts.append(pt[0])
-    spent, flops = float(sum(secs)), float(per_task[pick].sum())
-    model = "unknown CPU"
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
-    except (OSError, StopIteration):
-        pass
-    quart = [int(np.searchsorted(np.sort(per_task), per_task[k], side="right") * 4 //
-                 (len(per_task) + 1)) for k in pick]
-    return dict(rate=flops / spent, spent=spent, pick=pick, flops=flops, sampled=len(sample),
-                task_flops=[float(per_task[k]) for k in pick], task_s=secs, quartiles=quart,
-                threads=O.num_threads(), partials=np.asarray(parts), model=model,
-                nproc=os.cpu_count())
+    tasks (n_per_quartile drawn from each flop quartile, smallest first) in a
+    host thread beside the GPU work, which leaves the host cores idle while
+    the device runs.  The oracle runs on all host cores but one (the one the
+    GPU-driving thread uses).  A task starts only if it is predicted, at the
+    rate measured so far, to end before the deadline (seconds from process
+    start); at least one task runs."""
+
+    def __init__(self, tiling, s, per_task, deadline_s, n_per_quartile=2):
+        super().__init__(daemon=True)
+        self.tiling, self.s, self.per_task, self.deadline = tiling, s, per_task, deadline_s
+        order = np.argsort(per_task, kind="stable")
+        rng = np.random.default_rng(SEED)
+        strata = np.array_split(order, 4)
+        sample = np.concatenate([rng.choice(q, min(n_per_quartile, len(q)), replace=False)
+                                 for q in strata])
+        self.stratum = {int(k): i for i, q in enumerate(strata) for k in q}
+        self.sample = sample[np.argsort(per_task[sample], kind="stable")]
+        self.threads = max(1, (os.cpu_count() or 2) - 1)
+        self.pick, self.parts, self.secs, self.error = [], [], [], None
+
+    def run(self):
+        try:
+            from oracle import oracle as O
+            O.set_num_threads(self.threads)
+            occ, vir = self.tiling.occ, self.tiling.vir
+            t = O.Tiling(occ, occ, vir, vir)
+            evl = np.concatenate([self.s["eo"], self.s["eo"], self.s["ev"], self.s["ev"]])
+            rate = 5e9  # prior (flop/s) until the first task is timed
+            for k in self.sample:
+                if self.pick:
+                    rate = float(self.per_task[self.pick].sum()) / sum(self.secs)
+                    if elapsed() + float(self.per_task[k]) / rate > self.deadline:
+                        break
+                t0 = time.time()
+                _, _, pt = O.run_spatial(t, evl, self.s, True, [int(k)])
+                self.secs.append(time.time() - t0)
+                self.pick.append(int(k))
+                self.parts.append(pt[0])
+        except Exception as e:  # reported in the line, never silently
+            self.error = repr(e)
+
+    def result(self):
+        spent, flops = float(sum(self.secs)), float(self.per_task[self.pick].sum())
+        model = "unknown CPU"
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
+        quart = [self.stratum[k] for k in self.pick]
+        return dict(rate=flops / spent, spent=spent, pick=self.pick, flops=flops,
+                    sampled=len(self.sample), task_flops=[float(self.per_task[k]) for k in self.pick],
+                    task_s=self.secs, quartiles=quart, threads=self.threads,
+                    partials=np.asarray(self.parts), model=model, nproc=os.cpu_count())
 
 
 def main():
@@ -374,7 +395,9 @@ def main():
                          "GPU (xt_upload_cholesky) against numpy on the host cores; 0 skips it")
     ap.add_argument("--expand-leg", action="store_true",
                     help="also time the dense spin-orbital T1/T2 expansion (xt_expand_amplitudes)")
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--cpu-deadline-s", type=float, default=0.0,
+                    help="seconds from process start by which the CPU baseline stops starting "
+                         "tasks (default: the wall-clock budget less 30 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -399,6 +422,12 @@ def main():
     upload_s = time.time() - t0
     total_flops, per_task, parts = ctx.flops()
     n_tasks = ctx.num_tasks()
+    # the CPU baseline runs on the host cores beside all the GPU work below
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = CpuBaseline(tiling, s, per_task,
+                          args.cpu_deadline_s if args.cpu_deadline_s > 0 else args.budget_s - 30.0)
+        cpu.start()
     # operand conversion of the chosen precision (one-time, like the upload)
     t0 = time.time()
     ctx.prepare(prec)
@@ -535,30 +564,6 @@ def main():
         line["dropin_per_task"] = dropin_leg(xt, ctx, tiling, s, per_task, args.dropin_tasks)
         log(f"per-task drop-in: {line['dropin_per_task']}")
 
-    if not args.no_cpu_baseline:
-        cb = cpu_baseline(tiling, s, per_task, args.cpu_budget)
-        line["cpu_baseline"] = {
-            "value": total_flops / cb["rate"], "unit": "s (extrapolated time-to-E(T))",
-            "cores": cb["threads"], "kind": "port",
-            "sample": f"{len(cb['pick'])} of {cb['sampled']} seeded size-stratified tasks "
-                      f"(two per flop quartile, smallest first; quartiles {cb['quartiles']}, "
-                      f"{min(cb['task_flops']):.2e}-{max(cb['task_flops']):.2e} flops each, "
-                      f"{cb['flops']:.3e} in all) timed in {cb['spent']:.1f}s = "
-                      f"{cb['rate'] / 1e9:.2f} GFLOP/s on {cb['threads']} OpenMP threads "
-                      f"({cb['model']}, nproc {cb['nproc']}), extrapolated by the analytic "
-                      f"flop count",
-            "task_gflops": [round(fl / sec / 1e9, 3) for fl, sec in
-                            zip(cb["task_flops"], cb["task_s"])]}
-        chk = {}
-        for name, p in precs:
-            g = ctx.run_tasks(cb["pick"], precision=p)["task_partials"]
-            ref = cb["partials"]
-            chk[name] = {"max_abs_dE4": float(np.max(np.abs(g[:, 0] - ref[:, 0]))),
-                         "max_abs_dE5": float(np.max(np.abs(g[:, 1] - ref[:, 1]))),
-                         "max_rel_dE4": float(np.max(np.abs(g[:, 0] - ref[:, 0]) /
-                                                     np.abs(ref[:, 0])))}
-        line["cpu_sample_check"] = dict(tasks=cb["pick"], **chk)
-        log(f"cpu baseline: {line['cpu_baseline']}")
     del s
 
     # physical energy scale: the same tiling with integrals scaled down so
@@ -602,6 +607,38 @@ def main():
         if k + 1 < n_steps and elapsed() + 1.05 * times[-1] > args.budget_s:
             log("budget reached")
             break
+    if cpu is not None:
+        cpu.join()
+        if cpu.error or not cpu.pick:
+            raise RuntimeError(f"CPU baseline failed: {cpu.error}")
+        cb = cpu.result()
+        line["cpu_baseline"] = {
+            "value": total_flops / cb["rate"], "unit": "s (extrapolated time-to-E(T))",
+            "cores": cb["threads"], "kind": "port",
+            "sample": f"{len(cb['pick'])} of {cb['sampled']} seeded size-stratified tasks "
+                      f"(two per flop quartile, smallest first; quartiles {cb['quartiles']}, "
+                      f"{min(cb['task_flops']):.2e}-{max(cb['task_flops']):.2e} flops each, "
+                      f"{cb['flops']:.3e} in all) timed in {cb['spent']:.1f}s = "
+                      f"{cb['rate'] / 1e9:.2f} GFLOP/s on {cb['threads']} OpenMP threads "
+                      f"({cb['model']}, nproc {cb['nproc']}) in a host thread beside the GPU "
+                      f"work, extrapolated by the analytic flop count",
+            "tasks_timed": len(cb["pick"]),
+            "task_gflops": [round(fl / sec_ / 1e9, 3) for fl, sec_ in
+                            zip(cb["task_flops"], cb["task_s"])]}
+        chk = {}
+        for name, p in precs:
+            g = ctx.run_tasks(cb["pick"], precision=p)["task_partials"]
+            ref = cb["partials"]
+            chk[name] = {"max_abs_dE4": float(np.max(np.abs(g[:, 0] - ref[:, 0]))),
+                         "max_abs_dE5": float(np.max(np.abs(g[:, 1] - ref[:, 1]))),
+                         "max_rel_dE4": float(np.max(np.abs(g[:, 0] - ref[:, 0]) /
+                                                     np.abs(ref[:, 0])))}
+            if p == xt.PREC_FP64:
+                chk[name]["within_1e-10_plus_2e-14_rel"] = bool(
+                    np.all(np.abs(g - ref) <= 1e-10 + 2e-14 * np.abs(ref)))
+        line["cpu_sample_check"] = dict(tasks=cb["pick"], reference="oracle timed in this run",
+                                        **chk)
+        log(f"cpu baseline: {line['cpu_baseline']}")
     sec = float(np.mean(times))
     kernel_s = float(np.mean(kms)) / 1e3
     achieved = total_flops / kernel_s / 1e12

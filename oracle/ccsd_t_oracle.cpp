@@ -1291,4 +1291,14 @@ int oracle_num_threads() {
 #endif
 }
 
+// Threads of the calling thread's later parallel regions (OpenMP's per-thread
+// setting, so a host thread can time the oracle beside other work).
+void oracle_set_num_threads(int n) {
+#ifdef _OPENMP
+  if(n > 0) omp_set_num_threads(n);
+#else
+  (void) n;
+#endif
+}
+
 }  // extern "C"

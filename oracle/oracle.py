@@ -89,6 +89,8 @@ def lib():
         L.oracle_task_packed.argtypes = [C.c_int, C.c_int, ip, ip, C.c_int] + [dp] * 5 + [
             lp] + [dp] * 6 + [ip, ip]
         L.oracle_num_threads.restype = C.c_int
+        L.oracle_set_num_threads.argtypes = [C.c_int]
+        L.oracle_set_num_threads.restype = None
         L.oracle_gen_synth.restype = None
         L.oracle_gen_synth.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_double,
                                        C.c_double] + [dp] * 7
@@ -354,3 +356,8 @@ class SpatialGather:
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+def set_num_threads(n):
+    """OpenMP threads of the calling thread's later oracle calls."""
+    lib().oracle_set_num_threads(int(n))
