@@ -108,9 +108,11 @@ def lib():
     L.xt_gather_last_elems.argtypes = [vp, lp]
     L.xt_fully_fused_ccsd_t_gpu.argtypes = (
         [vp, sz] + [sz] * 6 + [vp] * 6 + [ip] * 6 + [sz] * 8 + [vp] * 6 + [vp, vp])
+    L.xt_fully_fused_ccsd_t_gpu_noalloc.argtypes = L.xt_fully_fused_ccsd_t_gpu.argtypes
     for name in ("xt_create", "xt_destroy", "xt_set_tiling", "xt_num_tasks", "xt_task_list",
                  "xt_flops", "xt_upload_closed_shell", "xt_synth_upload", "xt_synth_closed_shell",
-                 "xt_run", "xt_last_kernel_ms", "xt_fully_fused_ccsd_t_gpu", "xt_run_tasks",
+                 "xt_run", "xt_last_kernel_ms", "xt_fully_fused_ccsd_t_gpu",
+                 "xt_fully_fused_ccsd_t_gpu_noalloc", "xt_run_tasks",
                  "xt_sum_task_energies", "xt_memory_required",
                  "xt_task_codes", "xt_queue_layout", "xt_prepare", "xt_mlp_create", "xt_mlp_destroy",
                  "xt_mlp_get_params", "xt_mlp_set_params", "xt_mlp_loss_grad", "xt_mlp_train",

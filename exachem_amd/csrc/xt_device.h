@@ -203,6 +203,9 @@ int mlp_train_step(const MlpWork& W, long n, float lr, float* loss_hist, void* s
 // Tile-count limits of the per-task launcher (the reference's, nontc:56-57);
 // the host rejects larger noab / nvab before any launch.
 constexpr int kMaxNoab = 50, kMaxNvab = 140;
+// Ints of the launcher's packed host metadata at those limits: d1_size[7*noab],
+// d1_exec[9*noab], d2_size[7*nvab], d2_exec[9*nvab], s1_size[6], s1_exec[9].
+constexpr int kPackedMetaInts = 16 * kMaxNoab + 16 * kMaxNvab + 15;
 struct PackedArgs {
   int           dim[6];
   const double* d1_t2;
@@ -219,5 +222,13 @@ struct PackedArgs {
   long          num_blocks;
 };
 int launch_packed_fp64(const PackedArgs& A, void* stream);
+// The same launch without allocating: the n_meta ints of host_meta go into a
+// fixed device array of kPackedMetaInts ints (one per GPU, kept until the
+// process ends) through the arguments of small copy kernels on the stream,
+// and A.meta is set to that array; after_copy (a hipEvent_t, may be null) is
+// recorded between the copies and the launch.  Calls that overlap on one GPU
+// race on the array.
+int launch_packed_fp64_noalloc(PackedArgs& A, const int* host_meta, int n_meta, void* stream,
+                               void* after_copy);
 
 }  // namespace xt

@@ -1563,7 +1563,7 @@ int xt_learn_schedule(xt_ctx* c, xt_mlp* m, const xt_opts* opts, long n_sample, 
 namespace {
 using xt::kMaxNoab;
 using xt::kMaxNvab;
-constexpr int kMetaInts = 16 * kMaxNoab + 16 * kMaxNvab + 15;
+constexpr int kMetaInts = xt::kPackedMetaInts;
 constexpr int kRing     = 4;
 
 struct PackedState {
@@ -1584,7 +1584,83 @@ struct PackedState {
   }
 };
 thread_local PackedState g_packed;
+
+// Arguments of fully_fused_ccsd_t_gpu<double> (nontc:2800-2817) as the two
+// entry points receive them.
+struct PackedCall {
+  size_t        num_blocks, d[6];
+  const double *d1_t2, *d1_v2, *d2_t2, *d2_v2, *s1_t1, *s1_v2;
+  const int *   d1_size, *d1_exec, *d2_size, *d2_exec, *s1_size, *s1_exec;
+  size_t        noab, max_d1_t2, max_d1_v2, nvab, max_d2_t2, max_d2_v2, max_s1_t1, max_s1_v2;
+  const double* evl[6];
+  double*       partial;
+};
+
+// Checks a launcher call, packs its host size/exec arrays into meta
+// (kMetaInts ints: d1_size, d1_exec, d2_size, d2_exec, s1_size, s1_exec) and
+// fills the kernel arguments except the device copy of meta.
+int packed_prepare(const char* fn, const PackedCall& P, int* meta, int* n_meta, xt::PackedArgs& A) {
+  const std::string f(fn);
+  if(!P.d1_t2 || !P.d1_v2 || !P.d2_t2 || !P.d2_v2 || !P.s1_t1 || !P.s1_v2 || !P.d1_size ||
+     !P.d1_exec || !P.d2_size || !P.d2_exec || !P.s1_size || !P.s1_exec || !P.evl[0] ||
+     !P.evl[1] || !P.evl[2] || !P.evl[3] || !P.evl[4] || !P.evl[5] || !P.partial)
+    return fail(XT_EINVAL, f + ": null argument");
+  if(P.noab == 0 || P.nvab == 0 || P.noab > (size_t) kMaxNoab || P.nvab > (size_t) kMaxNvab)
+    return fail(XT_EINVAL, f + ": noab/nvab outside 1..50 / 1..140");
+  size_t nb = 1;
+  for(size_t x: P.d) {
+    if(x == 0 || x > 4096) return fail(XT_EINVAL, f + ": bad tile size");
+    nb *= (x + 3) / 4;
+  }
+  if(nb != P.num_blocks) return fail(XT_EINVAL, f + ": num_blocks != prod(ceil(dim/4))");
+  for(int i = 0; i < 6; i++)
+    if((size_t) P.s1_size[i] != P.d[i]) return fail(XT_EINVAL, f + ": s1_size disagrees with tile sizes");
+  for(size_t b = 0; b < P.noab; b++)
+    if(P.d1_size[7 * b + 3] <= 0 || P.d1_size[7 * b + 3] > 4096)
+      return fail(XT_EINVAL, f + ": bad h7 tile size");
+  for(size_t b = 0; b < P.nvab; b++)
+    if(P.d2_size[7 * b + 6] <= 0 || P.d2_size[7 * b + 6] > 4096)
+      return fail(XT_EINVAL, f + ": bad p7 tile size");
+  int* m   = meta;
+  auto put = [&](const int* src, size_t n) {
+    std::memcpy(m, src, n * sizeof(int));
+    m += n;
+  };
+  put(P.d1_size, 7 * P.noab);
+  put(P.d1_exec, 9 * P.noab);
+  put(P.d2_size, 7 * P.nvab);
+  put(P.d2_exec, 9 * P.nvab);
+  put(P.s1_size, 6);
+  put(P.s1_exec, 9);
+  *n_meta = (int) (m - meta);
+  for(int i = 0; i < 6; i++) A.dim[i] = (int) P.d[i];
+  A.d1_t2      = P.d1_t2;
+  A.d1_v2      = P.d1_v2;
+  A.d2_t2      = P.d2_t2;
+  A.d2_v2      = P.d2_v2;
+  A.s1_t1      = P.s1_t1;
+  A.s1_v2      = P.s1_v2;
+  A.meta       = nullptr;
+  A.noab       = (int) P.noab;
+  A.nvab       = (int) P.nvab;
+  A.max_d1_t2  = (long) P.max_d1_t2;
+  A.max_d1_v2  = (long) P.max_d1_v2;
+  A.max_d2_t2  = (long) P.max_d2_t2;
+  A.max_d2_v2  = (long) P.max_d2_v2;
+  A.max_s1_t1  = (long) P.max_s1_t1;
+  A.max_s1_v2  = (long) P.max_s1_v2;
+  for(int i = 0; i < 6; i++) A.evl[i] = P.evl[i];
+  A.partial    = P.partial;
+  A.num_blocks = (long) P.num_blocks;
+  return XT_OK;
+}
 }  // namespace
+
+#define XT_PACKED_CALL                                                                              \
+  PackedCall P = {num_blocks, {h1, h2, h3, p4, p5, p6}, d1_t2, d1_v2, d2_t2, d2_v2, s1_t1, s1_v2,  \
+                  d1_size, d1_exec, d2_size, d2_exec, s1_size, s1_exec, noab, max_d1_t2,          \
+                  max_d1_v2, nvab, max_d2_t2, max_d2_v2, max_s1_t1, max_s1_v2,                    \
+                  {evl_h1, evl_h2, evl_h3, evl_p4, evl_p5, evl_p6}, partial}
 
 int xt_fully_fused_ccsd_t_gpu(void* stream, size_t num_blocks, size_t h1, size_t h2, size_t h3,
                               size_t p4, size_t p5, size_t p6, const double* d1_t2,
@@ -1597,23 +1673,13 @@ int xt_fully_fused_ccsd_t_gpu(void* stream, size_t num_blocks, size_t h1, size_t
                               const double* evl_h1, const double* evl_h2, const double* evl_h3,
                               const double* evl_p4, const double* evl_p5, const double* evl_p6,
                               double* partial, void* done_copy) {
-  if(!d1_t2 || !d1_v2 || !d2_t2 || !d2_v2 || !s1_t1 || !s1_v2 || !d1_size || !d1_exec ||
-     !d2_size || !d2_exec || !s1_size || !s1_exec || !evl_h1 || !evl_h2 || !evl_h3 || !evl_p4 ||
-     !evl_p5 || !evl_p6 || !partial)
-    return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: null argument");
-  if(noab == 0 || nvab == 0 || noab > (size_t) kMaxNoab || nvab > (size_t) kMaxNvab)
-    return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: noab/nvab outside 1..50 / 1..140");
-  const size_t d[6] = {h1, h2, h3, p4, p5, p6};
-  size_t       nb   = 1;
-  for(size_t x: d) {
-    if(x == 0 || x > 4096) return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: bad tile size");
-    nb *= (x + 3) / 4;
-  }
-  if(nb != num_blocks)
-    return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: num_blocks != prod(ceil(dim/4))");
-  for(int i = 0; i < 6; i++)
-    if((size_t) s1_size[i] != d[i])
-      return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: s1_size disagrees with tile sizes");
+  XT_PACKED_CALL;
+  // validate before touching any device state
+  int            staged[kMetaInts];
+  int            n_meta = 0;
+  xt::PackedArgs A;
+  int            rc = packed_prepare("xt_fully_fused_ccsd_t_gpu", P, staged, &n_meta, A);
+  if(rc) return rc;
   PackedState& S = g_packed;
   int          dev;
   XT_HIP(hipGetDevice(&dev));
@@ -1629,54 +1695,41 @@ int xt_fully_fused_ccsd_t_gpu(void* stream, size_t num_blocks, size_t h1, size_t
   S.slot           = (S.slot + 1) % kRing;
   XT_HIP(hipEventSynchronize(S.ev[slot]));  // previous use of this ring slot finished
   int* hm = S.pinned + slot * kMetaInts;
-  int* m  = hm;
-  auto put = [&](const int* src, size_t n) {
-    std::memcpy(m, src, n * sizeof(int));
-    m += n;
-  };
-  put(d1_size, 7 * noab);
-  put(d1_exec, 9 * noab);
-  put(d2_size, 7 * nvab);
-  put(d2_exec, 9 * nvab);
-  put(s1_size, 6);
-  put(s1_exec, 9);
-  for(size_t b = 0; b < noab; b++)
-    if(d1_size[7 * b + 3] <= 0 || d1_size[7 * b + 3] > 4096)
-      return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: bad h7 tile size");
-  for(size_t b = 0; b < nvab; b++)
-    if(d2_size[7 * b + 6] <= 0 || d2_size[7 * b + 6] > 4096)
-      return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu: bad p7 tile size");
+  std::memcpy(hm, staged, sizeof(int) * (size_t) n_meta);
   int* dm = S.dmeta + slot * kMetaInts;
-  XT_HIP(hipMemcpyAsync(dm, hm, (size_t) (m - hm) * sizeof(int), hipMemcpyHostToDevice, st));
+  XT_HIP(hipMemcpyAsync(dm, hm, (size_t) n_meta * sizeof(int), hipMemcpyHostToDevice, st));
   if(done_copy) XT_HIP(hipEventRecord((hipEvent_t) done_copy, st));
-  xt::PackedArgs A;
-  for(int i = 0; i < 6; i++) A.dim[i] = (int) d[i];
-  A.d1_t2      = d1_t2;
-  A.d1_v2      = d1_v2;
-  A.d2_t2      = d2_t2;
-  A.d2_v2      = d2_v2;
-  A.s1_t1      = s1_t1;
-  A.s1_v2      = s1_v2;
-  A.meta       = dm;
-  A.noab       = (int) noab;
-  A.nvab       = (int) nvab;
-  A.max_d1_t2  = (long) max_d1_t2;
-  A.max_d1_v2  = (long) max_d1_v2;
-  A.max_d2_t2  = (long) max_d2_t2;
-  A.max_d2_v2  = (long) max_d2_v2;
-  A.max_s1_t1  = (long) max_s1_t1;
-  A.max_s1_v2  = (long) max_s1_v2;
-  A.evl[0]     = evl_h1;
-  A.evl[1]     = evl_h2;
-  A.evl[2]     = evl_h3;
-  A.evl[3]     = evl_p4;
-  A.evl[4]     = evl_p5;
-  A.evl[5]     = evl_p6;
-  A.partial    = partial;
-  A.num_blocks = (long) num_blocks;
-  int err      = xt::launch_packed_fp64(A, st);
+  A.meta  = dm;
+  int err = xt::launch_packed_fp64(A, st);
   if(err) return fail(XT_EHIP, std::string("packed launch: ") + hipGetErrorString((hipError_t) err));
   XT_HIP(hipEventRecord(S.ev[slot], st));
+  return XT_OK;
+}
+
+int xt_fully_fused_ccsd_t_gpu_noalloc(
+  void* stream, size_t num_blocks, size_t h1, size_t h2, size_t h3, size_t p4, size_t p5, size_t p6,
+  const double* d1_t2, const double* d1_v2, const double* d2_t2, const double* d2_v2,
+  const double* s1_t1, const double* s1_v2, const int* d1_size, const int* d1_exec,
+  const int* d2_size, const int* d2_exec, const int* s1_size, const int* s1_exec, size_t noab,
+  size_t max_d1_t2, size_t max_d1_v2, size_t nvab, size_t max_d2_t2, size_t max_d2_v2,
+  size_t max_s1_t1, size_t max_s1_v2, const double* evl_h1, const double* evl_h2,
+  const double* evl_h3, const double* evl_p4, const double* evl_p5, const double* evl_p6,
+  double* partial, void* done_copy) {
+  XT_PACKED_CALL;
+  int            meta[kMetaInts];  // on the stack: nothing is allocated
+  int            n_meta = 0;
+  xt::PackedArgs A;
+  int rc = packed_prepare("xt_fully_fused_ccsd_t_gpu_noalloc", P, meta, &n_meta, A);
+  if(rc) return rc;
+  if(n_meta > kMetaInts)  // cannot happen inside the noab/nvab limits checked above
+    return fail(XT_EINVAL, "xt_fully_fused_ccsd_t_gpu_noalloc: metadata beyond the scratch size");
+  // the metadata goes to the device in the copy kernels' arguments, so the
+  // caller's arrays are free once the call returns; done_copy is recorded
+  // after those copies, before the launch
+  int err = xt::launch_packed_fp64_noalloc(A, meta, n_meta, stream, done_copy);
+  if(err)
+    return fail(XT_EHIP, std::string("xt_fully_fused_ccsd_t_gpu_noalloc: ") +
+                           hipGetErrorString((hipError_t) err));
   return XT_OK;
 }
 

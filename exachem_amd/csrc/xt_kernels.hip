@@ -20,6 +20,10 @@
 //     with wave64 shuffles and a fixed-order cross-wave sum.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
 #include "xt_device.h"
 #include "xt_fixed.h"
 #include "xt_queue.h"
@@ -800,6 +804,72 @@ int launch_packed_fp64(const PackedArgs& A, void* stream) {
   hipLaunchKernelGGL(k_packed_fp64, dim3((unsigned) A.num_blocks), dim3(NT), 0,
                      (hipStream_t) stream, A);
   return (int) hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form of the per-task launch.  The launcher metadata lives in
+// `scratch`, one fixed array per GPU for the life of the process.  In host
+// code the name `scratch` is only a stand-in for the symbol, so every host use
+// goes through the device address hipGetSymbolAddress returns, looked up once
+// per device (with that device current) and kept in scratch_ptr.  The host
+// arrays reach it through the arguments of k_store_meta launches on the
+// caller's stream: ordered with the caller's work, captured by value at
+// launch (the caller may reuse its arrays at once), and no allocation, copy
+// engine or memset node involved.  Calls that overlap on one GPU race on it.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) int scratch[kPackedMetaInts];
+
+namespace {
+constexpr int kMetaChunk = 960;  // ints per copy launch (3.8 KB of kernel arguments)
+struct MetaChunk {
+  int off, n;
+  int v[kMetaChunk];
+};
+
+__global__ void __launch_bounds__(256) k_store_meta(int* dst, MetaChunk c) {
+  for(int i = threadIdx.x; i < c.n; i += blockDim.x) dst[c.off + i] = c.v[i];
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex    g_scratch_mu;
+int*          scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's scratch; nullptr on failure (*err set).
+int* scratch_on_current_device(int* err) {
+  int dev = 0;
+  if((*err = (int) hipGetDevice(&dev))) return nullptr;
+  if(dev < 0 || dev >= kMaxDevices) {
+    *err = (int) hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if(!scratch_ptr[dev]) {
+    void* p = nullptr;
+    if((*err = (int) hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)))) return nullptr;
+    scratch_ptr[dev] = static_cast<int*>(p);
+  }
+  return scratch_ptr[dev];
+}
+}  // namespace
+
+int launch_packed_fp64_noalloc(PackedArgs& A, const int* host_meta, int n_meta, void* stream,
+                               void* after_copy) {
+  if(n_meta < 0 || n_meta > kPackedMetaInts) return (int) hipErrorInvalidValue;
+  int  err = 0;
+  int* dst = scratch_on_current_device(&err);
+  if(!dst) return err ? err : (int) hipErrorInvalidSymbol;
+  hipStream_t st = (hipStream_t) stream;
+  MetaChunk   c  = {};
+  for(int off = 0; off < n_meta; off += kMetaChunk) {
+    c.off = off;
+    c.n   = std::min(kMetaChunk, n_meta - off);
+    std::memcpy(c.v, host_meta + off, sizeof(int) * (size_t) c.n);
+    hipLaunchKernelGGL(k_store_meta, dim3(1), dim3(256), 0, st, dst, c);
+    if((err = (int) hipGetLastError())) return err;
+  }
+  if(after_copy && (err = (int) hipEventRecord((hipEvent_t) after_copy, st))) return err;
+  A.meta = dst;
+  return launch_packed_fp64(A, stream);
 }
 
 }  // namespace xt

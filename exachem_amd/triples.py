@@ -412,7 +412,7 @@ class CCSDTriples:
 
 
 def fully_fused_ccsd_t_gpu(stream, num_blocks, dims, dev_bufs, meta, noab, nvab, max_dims, evl,
-                           partial, done_copy=None):
+                           partial, done_copy=None, noalloc=False):
     """Per-task drop-in for ``fully_fused_ccsd_t_gpu<double>``
     (ccsd_t_all_fused_nontcCuda_Hip_Sycl.cpp:2800-2817).
 
@@ -420,10 +420,12 @@ def fully_fused_ccsd_t_gpu(stream, num_blocks, dims, dev_bufs, meta, noab, nvab,
     (d1_t2, d1_v2, d2_t2, d2_v2, s1_t1, s1_v2); meta: dict of int32 host arrays
     d1_size, d1_exec, d2_size, d2_exec, s1_size, s1_exec; max_dims: dict with
     d1_t2, d1_v2, d2_t2, d2_v2, s1_t1, s1_v2 (doubles per slot); evl: 6 device
-    pointers; partial: device pointer of 2*num_blocks doubles."""
+    pointers; partial: device pointer of 2*num_blocks doubles.  noalloc: the
+    allocation-free entry point (xt_fully_fused_ccsd_t_gpu_noalloc)."""
     m = {k: np.ascontiguousarray(meta[k], dtype=np.int32) for k in
          ("d1_size", "d1_exec", "d2_size", "d2_exec", "s1_size", "s1_exec")}
-    return check(lib().xt_fully_fused_ccsd_t_gpu(
+    fn = lib().xt_fully_fused_ccsd_t_gpu_noalloc if noalloc else lib().xt_fully_fused_ccsd_t_gpu
+    return check(fn(
         stream, num_blocks, *dims, *dev_bufs,
         _ip(m["d1_size"]), _ip(m["d1_exec"]), _ip(m["d2_size"]), _ip(m["d2_exec"]),
         _ip(m["s1_size"]), _ip(m["s1_exec"]),

@@ -94,6 +94,34 @@ int xt_fully_fused_ccsd_t_gpu(void* stream, size_t num_blocks, size_t base_size_
                               const double* dev_evl_sorted_p6b, double* partial_energies,
                               void* done_copy);
 
+/* The per-task launcher above without any allocation: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory, such as
+ * code under HIP graph capture, tight serving loops or harnesses that forbid
+ * hipMalloc.  Same arguments, meaning, checks and results (bit for bit) as
+ * xt_fully_fused_ccsd_t_gpu; it calls no hipMalloc, hipFree or other
+ * allocating routine.  The launcher's metadata goes into one fixed device
+ * array per GPU of 16*50 + 16*140 + 15 ints (12,220 bytes: the reference's
+ * MAX_NOAB = 50 / MAX_NVAB = 140 limits, larger noab / nvab fail with
+ * XT_EINVAL), held until the process ends and written by small copy kernels
+ * on `stream` whose arguments carry the host arrays, so the caller may reuse
+ * them when the call returns (done_copy is recorded after those copies).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (different streams or host threads, without ordering between them) race. */
+int xt_fully_fused_ccsd_t_gpu_noalloc(
+  void* stream, size_t num_blocks, size_t base_size_h1b, size_t base_size_h2b,
+  size_t base_size_h3b, size_t base_size_p4b, size_t base_size_p5b, size_t base_size_p6b,
+  const double* df_dev_d1_t2_all, const double* df_dev_d1_v2_all,
+  const double* df_dev_d2_t2_all, const double* df_dev_d2_v2_all,
+  const double* df_dev_s1_t1_all, const double* df_dev_s1_v2_all, const int* host_d1_size,
+  const int* host_d1_exec, const int* host_d2_size, const int* host_d2_exec,
+  const int* host_s1_size, const int* host_s1_exec, size_t size_noab, size_t size_max_dim_d1_t2,
+  size_t size_max_dim_d1_v2, size_t size_nvab, size_t size_max_dim_d2_t2,
+  size_t size_max_dim_d2_v2, size_t size_max_dim_s1_t1, size_t size_max_dim_s1_v2,
+  const double* dev_evl_sorted_h1b, const double* dev_evl_sorted_h2b,
+  const double* dev_evl_sorted_h3b, const double* dev_evl_sorted_p4b,
+  const double* dev_evl_sorted_p5b, const double* dev_evl_sorted_p6b, double* partial_energies,
+  void* done_copy);
+
 /* --------------------------------------------- 2. whole-problem path */
 typedef struct xt_ctx xt_ctx;
 
