@@ -203,9 +203,18 @@ int mlp_train_step(const MlpWork& W, long n, float lr, float* loss_hist, void* s
 // Tile-count limits of the per-task launcher (the reference's, nontc:56-57);
 // the host rejects larger noab / nvab before any launch.
 constexpr int kMaxNoab = 50, kMaxNvab = 140;
-// Ints of the launcher's packed host metadata at those limits: d1_size[7*noab],
-// d1_exec[9*noab], d2_size[7*nvab], d2_exec[9*nvab], s1_size[6], s1_exec[9].
-constexpr int kPackedMetaInts = 16 * kMaxNoab + 16 * kMaxNvab + 15;
+// The launcher's packed metadata, in ints: the host arrays d1_size[7*noab],
+// d1_exec[9*noab], d2_size[7*nvab], d2_exec[9*nvab], s1_size[6], s1_exec[9],
+// then the K tile lists the host builds from them: kKlistCountInts counts
+// (one per bipartition, then padding to a 16-byte boundary) and, for
+// bipartition g = 0..8 in turn, count[g] + 1 int4 entries (family 0 d1 / 1
+// d2, exec slot, K, first packed k; the last entry holds the total K in .w).
+// kPackedMetaInts is the size at the reference's tile-count limits.
+constexpr int kKlistCountInts = 9;  // 15 + 9 ints after the size/exec arrays: 16-byte aligned
+constexpr int kPackedMetaInts =
+  16 * kMaxNoab + 16 * kMaxNvab + 15 + kKlistCountInts + 4 * 9 * (kMaxNoab + kMaxNvab + 1);
+static_assert((15 + kKlistCountInts) % 4 == 0, "tile lists must start on a 16-byte boundary");
+static_assert(kPackedMetaInts % 4 == 0, "metadata slots must keep 16-byte alignment");
 struct PackedArgs {
   int           dim[6];
   const double* d1_t2;

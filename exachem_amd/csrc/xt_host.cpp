@@ -1632,6 +1632,39 @@ int packed_prepare(const char* fn, const PackedCall& P, int* meta, int* n_meta, 
   put(P.d2_exec, 9 * P.nvab);
   put(P.s1_size, 6);
   put(P.s1_exec, 9);
+  // K tile lists of the nine bipartitions g = 3*hs + ps: the enabled d1
+  // tiles (h7b) with their exec slot for d1 equation 3*((ps+1)%3) + hs, then
+  // the enabled d2 tiles (p7b) for d2 equation 3*ps + (hs+1)%3, packed end to
+  // end (ccsd_t_all_fused_cpu.hpp:173-270, 331-428)
+  int* counts = m;
+  m += xt::kKlistCountInts;
+  for(int g = 0; g < 9; g++) {
+    const int hs = g / 3, ps = g % 3, s1d = 3 * ((ps + 1) % 3) + hs, s2d = 3 * ps + (hs + 1) % 3;
+    int       n = 0, kb = 0;
+    auto      entry = [&](int fam, int ex, int K, int k0) {
+      m[0] = fam;
+      m[1] = ex;
+      m[2] = K;
+      m[3] = k0;
+      m += 4;
+    };
+    for(size_t tb = 0; tb < P.noab; tb++) {
+      const int ex = P.d1_exec[s1d + 9 * tb];
+      if(ex < 0) continue;
+      entry(0, ex, P.d1_size[3 + 7 * tb], kb);
+      kb += P.d1_size[3 + 7 * tb];
+      n++;
+    }
+    for(size_t tb = 0; tb < P.nvab; tb++) {
+      const int ex = P.d2_exec[s2d + 9 * tb];
+      if(ex < 0) continue;
+      entry(1, ex, P.d2_size[6 + 7 * tb], kb);
+      kb += P.d2_size[6 + 7 * tb];
+      n++;
+    }
+    entry(0, 0, 0, kb);
+    counts[g] = n;
+  }
   *n_meta = (int) (m - meta);
   for(int i = 0; i < 6; i++) A.dim[i] = (int) P.d[i];
   A.d1_t2      = P.d1_t2;

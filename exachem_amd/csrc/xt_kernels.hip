@@ -626,7 +626,6 @@ int launch_reduce(const TaskDev* tasks, int n_tasks, const unsigned long long* a
 __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
   __shared__ Smem sm;
   __shared__ int4 klist[kMaxNoab + kMaxNvab + 1];  // tiles of one bipartition's K, + sentinel
-  __shared__ int  s_nk;
   const int       tid = threadIdx.x, row = tid >> 2, kq = tid & 3;
   const int*      dim = A.dim;
   int             nsub[6], sub[6];
@@ -645,6 +644,9 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
   const int* d2_size = d1_exec + 9 * A.noab;
   const int* d2_exec = d2_size + 7 * A.nvab;
   const int* s1_exec = d2_exec + 9 * A.nvab + 6;
+  // the host's K tile lists of the nine bipartitions (packed_klists)
+  const int*  k_n    = s1_exec + 9;
+  const int4* k_list = reinterpret_cast<const int4*>(k_n + kKlistCountInts);
 
   for(int e = tid; e < T3N; e += NT) sm.t3[e] = 0.0;
   if(tid < 24) {
@@ -679,35 +681,22 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
       for(int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
     // K of this bipartition: the enabled d1 tiles (h7b), then the enabled d2
     // tiles (p7b), packed end to end and cut into chunks of KC, so that only
-    // the last chunk is padded.  Thread 0 lists the tiles (family, exec slot,
+    // the last chunk is padded.  The host lists the tiles (family, exec slot,
     // K, first packed k) with a sentinel holding the total; each thread walks
     // the list with its own tile index, which only moves forward since its k
-    // only grow.  The next chunk is loaded into registers while the current
-    // one is multiplied.  Every thread has read the previous bipartition's
-    // list and count before thread 0 rewrites them.
+    // only grow.  Every load is issued (rows and k outside the tile read the
+    // buffer's first element) and masked after it arrives, so the loads of a
+    // chunk are in flight together; the next chunk is loaded into registers
+    // while the current one is multiplied.  Every thread has read the
+    // previous bipartition's list before it is rewritten.
+    int koff = 0;
+    for(int gg = 0; gg < g; gg++) koff += k_n[gg] + 1;
+    const int nk = k_n[g];
     __syncthreads();
-    if(tid == 0) {
-      int n = 0, kb = 0;
-      for(int tb = 0; tb < A.noab; tb++) {
-        const int ex = d1_exec[s1d + 9 * tb];
-        if(ex >= 0) {
-          klist[n++] = make_int4(0, ex, d1_size[3 + 7 * tb], kb);
-          kb += d1_size[3 + 7 * tb];
-        }
-      }
-      for(int tb = 0; tb < A.nvab; tb++) {
-        const int ex = d2_exec[s2d + 9 * tb];
-        if(ex >= 0) {
-          klist[n++] = make_int4(1, ex, d2_size[6 + 7 * tb], kb);
-          kb += d2_size[6 + 7 * tb];
-        }
-      }
-      klist[n] = make_int4(0, 0, 0, kb);
-      s_nk     = n;
-    }
+    for(int x = tid; x <= nk; x += NT) klist[x] = k_list[koff + x];
     __syncthreads();
-    const int  nk = s_nk, kall = klist[nk].w;
-    const bool any = nk > 0;
+    const int  kall = klist[nk].w;
+    const bool any  = nk > 0;
     // operand rows of this thread: d1_t2 and d2_v2 / d2_t2 rows are times the
     // tile's K; the d1_v2 element of k sits at rB1 + k * bstr1
     const long rA1 = gPA + (gPB + gHS * DPB) * DPA, rB1 = gHB + (gHA + gPS * DHA) * DHB;
@@ -718,27 +707,41 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
     int        nbeg = klist[1].w;
     double     av[8], bv[8];
     auto       load = [&](int c) {
+      const double* pa[8];
+      const double* pb[8];
+      bool          ok[8];
 #pragma unroll
       for(int j = 0; j < 8; j++) {
         const int k = c * KC + 8 * kq + j;
-        double    a = 0.0, b = 0.0;
-        if(k < kall) {
+        ok[j]       = k < kall;
+        if(ok[j]) {
           while(k >= nbeg) {
             te   = klist[++ti];
             nbeg = klist[ti + 1].w;
           }
-          const long kl = k - te.w, K = te.z;
-          if(te.x == 0) {
-            if(mok) a = sg * A.d1_t2[A.max_d1_t2 * (long) te.y + rA1 * K + kl];
-            if(nok) b = A.d1_v2[A.max_d1_v2 * (long) te.y + rB1 + kl * bstr1];
-          }
-          else {
-            if(mok) a = sg * A.d2_v2[A.max_d2_v2 * (long) te.y + rA2 * K + kl];
-            if(nok) b = A.d2_t2[A.max_d2_t2 * (long) te.y + rB2 * K + kl];
-          }
         }
-        av[j] = a;
-        bv[j] = b;
+        const long kl = k - te.w, K = te.z;
+        if(te.x == 0) {
+          pa[j] = A.d1_t2 + A.max_d1_t2 * (long) te.y + rA1 * K + kl;
+          pb[j] = A.d1_v2 + A.max_d1_v2 * (long) te.y + rB1 + kl * bstr1;
+        }
+        else {
+          pa[j] = A.d2_v2 + A.max_d2_v2 * (long) te.y + rA2 * K + kl;
+          pb[j] = A.d2_t2 + A.max_d2_t2 * (long) te.y + rB2 * K + kl;
+        }
+        // rows outside the block and k past the end read a valid element
+        pa[j] = (ok[j] && mok) ? pa[j] : A.d1_t2;
+        pb[j] = (ok[j] && nok) ? pb[j] : A.d1_v2;
+      }
+#pragma unroll
+      for(int j = 0; j < 8; j++) {
+        av[j] = *pa[j];
+        bv[j] = *pb[j];
+      }
+#pragma unroll
+      for(int j = 0; j < 8; j++) {
+        av[j] = (ok[j] && mok) ? sg * av[j] : 0.0;
+        bv[j] = (ok[j] && nok) ? bv[j] : 0.0;
       }
     };
     if(any) {

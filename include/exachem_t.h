@@ -99,10 +99,11 @@ int xt_fully_fused_ccsd_t_gpu(void* stream, size_t num_blocks, size_t base_size_
  * code under HIP graph capture, tight serving loops or harnesses that forbid
  * hipMalloc.  Same arguments, meaning, checks and results (bit for bit) as
  * xt_fully_fused_ccsd_t_gpu; it calls no hipMalloc, hipFree or other
- * allocating routine.  The launcher's metadata goes into one fixed device
- * array per GPU of 16*50 + 16*140 + 15 ints (12,220 bytes: the reference's
- * MAX_NOAB = 50 / MAX_NVAB = 140 limits, larger noab / nvab fail with
- * XT_EINVAL), held until the process ends and written by small copy kernels
+ * allocating routine.  The launcher's metadata (the host size/exec arrays
+ * and the K tile lists built from them) goes into one fixed device array per
+ * GPU of 9,940 ints (39,760 bytes: its size at the reference's MAX_NOAB = 50
+ * / MAX_NVAB = 140 limits, larger noab / nvab fail with XT_EINVAL), held
+ * until the process ends and written by small copy kernels
  * on `stream` whose arguments carry the host arrays, so the caller may reuse
  * them when the call returns (done_copy is recorded after those copies).
  * Every call on one GPU shares that array: calls that overlap on one GPU
