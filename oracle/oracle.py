@@ -45,7 +45,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libccsd_t_oracle.so")
+# XT_ORACLE_LIB points the checker at another build of the same source (the
+# sanitizer build of the CPU tests)
+_LIB_PATH = os.environ.get("XT_ORACLE_LIB") or os.path.join(_HERE, "_build", "libccsd_t_oracle.so")
 _lib = None
 
 
