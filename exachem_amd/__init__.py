@@ -82,6 +82,7 @@ def lib():
     L.xt_run_tasks.argtypes = [vp, C.POINTER(XtOpts), C.c_long, lp, dp, dp, C.POINTER(C.c_float)]
     L.xt_sum_task_energies.argtypes = [C.c_long, dp, dp]
     L.xt_memory_required.argtypes = [vp, C.c_long, dp]
+    L.xt_upload_peak_bytes.argtypes = [vp, dp]
     L.xt_task_codes.argtypes = [vp, C.c_long, C.POINTER(C.c_uint64)]
     llp = C.POINTER(C.c_longlong)
     L.xt_queue_layout.argtypes = [vp, C.POINTER(XtOpts), C.c_long, lp, ip, ip, llp, llp, ip, ip]
@@ -113,7 +114,7 @@ def lib():
                  "xt_flops", "xt_upload_closed_shell", "xt_synth_upload", "xt_synth_closed_shell",
                  "xt_run", "xt_last_kernel_ms", "xt_fully_fused_ccsd_t_gpu",
                  "xt_fully_fused_ccsd_t_gpu_noalloc", "xt_run_tasks",
-                 "xt_sum_task_energies", "xt_memory_required",
+                 "xt_sum_task_energies", "xt_memory_required", "xt_upload_peak_bytes",
                  "xt_task_codes", "xt_queue_layout", "xt_prepare", "xt_mlp_create", "xt_mlp_destroy",
                  "xt_mlp_get_params", "xt_mlp_set_params", "xt_mlp_loss_grad", "xt_mlp_train",
                  "xt_mlp_predict", "xt_task_features", "xt_set_task_cost", "xt_learn_schedule",

@@ -19,7 +19,7 @@ struct Freer {
   void* p[4] = {nullptr, nullptr, nullptr, nullptr};
   ~Freer() {
     for(void* x: p)
-      if(x) (void) hipFree(x);
+      if(x) track_free(x);
   }
 };
 int blas_err(rocblas_status s) { return s == rocblas_status_success ? 0 : (int) hipErrorUnknown; }
@@ -33,11 +33,11 @@ int cholesky_integrals(int O, int V, long K, const double* L, double* vvoo, doub
   Freer        f;
   double *     lvo = nullptr, *lov = nullptr, *loo = nullptr, *tmp = nullptr;
   hipError_t   e;
-  if((e = hipMalloc((void**) &lvo, d * v * o * K))) return (int) e;
+  if((e = (hipError_t) track_malloc((void**) &lvo, d * v * o * K))) return (int) e;
   f.p[0] = lvo;
-  if((e = hipMalloc((void**) &lov, d * o * v * K))) return (int) e;
+  if((e = (hipError_t) track_malloc((void**) &lov, d * o * v * K))) return (int) e;
   f.p[1] = lov;
-  if((e = hipMalloc((void**) &loo, d * o * o * K))) return (int) e;
+  if((e = (hipError_t) track_malloc((void**) &loo, d * o * o * K))) return (int) e;
   f.p[2] = loo;
   // compact row sets: L_vo[(a,i)] = L[O+a][i], L_ov[(i,a)] = L[i][O+a], L_oo[(i,j)] = L[i][j]
   if((e = hipMemcpy2DAsync(lvo, d * o * K, L + o * nmo * K, d * nmo * K, d * o * K, v,
@@ -66,7 +66,7 @@ int cholesky_integrals(int O, int V, long K, const double* L, double* vvoo, doub
                               (int) (o * v), ik, &one, loo, ik, lov, ik, &zero, ovoo, (int) (o * o)));
   if(rc) return rc;
   // (ai|bj) over rows (a,i), columns (b,j), then into vvoo[a][b][i][j]
-  if((e = hipMalloc((void**) &tmp, d * v * o * v * o))) return (int) e;
+  if((e = (hipError_t) track_malloc((void**) &tmp, d * v * o * v * o))) return (int) e;
   f.p[3] = tmp;
   rc = blas_err(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (int) (v * o),
                               (int) (v * o), ik, &one, lvo, ik, lvo, ik, &zero, tmp, (int) (v * o)));

@@ -116,6 +116,13 @@ struct RunDev {
   int                 glog[6];  // ticket group shape (log2 extent per position, sum 6)
 };
 
+// Device allocations of the upload paths (xt_host.cpp): hipMalloc / hipFree,
+// also counted (live bytes and their maximum) while the calling thread's
+// upload has a count open, so that the upload's peak can be compared with the
+// memory guard's estimate (xt_upload_peak_bytes).  Returns hipError_t as int.
+int  track_malloc(void** p, size_t bytes);
+void track_free(void* p);
+
 // Kernel launchers (defined in xt_kernels.hip).  Return hipError_t as int.
 // Builds the padded / antisymmetrised operand arrays from the caller's
 // device copies t2 [V][V][O][O], ovoo [O][V][O][O] and vvvo [V][V][V][O]
@@ -206,10 +213,13 @@ constexpr int kMaxNoab = 50, kMaxNvab = 140;
 // The launcher's packed metadata, in ints: the host arrays d1_size[7*noab],
 // d1_exec[9*noab], d2_size[7*nvab], d2_exec[9*nvab], s1_size[6], s1_exec[9],
 // then the K tile lists the host builds from them: kKlistCountInts counts
-// (one per bipartition, then padding to a 16-byte boundary) and, for
-// bipartition g = 0..8 in turn, count[g] + 1 int4 entries (family 0 d1 / 1
-// d2, exec slot, K, first packed k; the last entry holds the total K in .w).
+// (one per bipartition: entries | d1 entries << 16, then padding to a
+// 16-byte boundary) and, for bipartition g = 0..8 in turn, entries + 1 int4
+// (family 0 d1 / 1 d2, exec slot, K, first packed k; the last entry holds the
+// total K in .w).  The first d2 tile starts on a multiple of kPackedKC, the
+// kernel's K chunk, so that every chunk reads one family.
 // kPackedMetaInts is the size at the reference's tile-count limits.
+constexpr int kPackedKC       = 32;
 constexpr int kKlistCountInts = 9;  // 15 + 9 ints after the size/exec arrays: 16-byte aligned
 constexpr int kPackedMetaInts =
   16 * kMaxNoab + 16 * kMaxNvab + 15 + kKlistCountInts + 4 * 9 * (kMaxNoab + kMaxNvab + 1);

@@ -15,6 +15,7 @@
 #include <numeric>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/exachem_t.h"
@@ -331,6 +332,7 @@ struct xt_ctx {
   long                          gather_elems = 0;  // doubles written by the last gather
   hipEvent_t          ev0 = nullptr, ev1 = nullptr;
   double              last_ms = 0.0;
+  double              upload_peak = 0.0;  // largest device bytes the last upload held at once
   std::vector<double> learned_cost;  // per-task predicted cost (learned schedule)
   // ticket group shapes (log2 extents per position), set by the autotuner
   int glog_fp64[6]  = {1, 1, 1, 1, 1, 1};
@@ -558,7 +560,10 @@ MemReq mem_req(size_t O, size_t V, long nchol) {
             2 * std::max(v * v * o, o * o * v);
   const double t2 = 8 * v * v * o * o, ov = 8 * o * o * o * v, xv = 8 * v * v * o * vp;
   if(nchol <= 0) {
-    r.staging = t2 + ov + 8 * v * v * v * o;  // t2, ovoo, vvvo as uploaded
+    // t2, ovoo and vvvo as uploaded; vvvo is freed once xv exists, before xva
+    // (at least as large) is allocated, so the peak holds t2 and ovoo beside
+    // all the operands
+    r.staging = t2 + ov + std::max(0.0, 8 * v * v * v * o - xv);
   }
   else {
     const double nmo = o + v, k = (double) nchol;
@@ -587,6 +592,12 @@ int check_free(const char* what, double a, const char* na, double b, const char*
   return fail(XT_ENOMEM, buf);
 }
 }  // namespace
+
+int xt_upload_peak_bytes(xt_ctx* c, double* bytes) {
+  if(!c || !bytes) return fail(XT_EINVAL, "xt_upload_peak_bytes: null argument");
+  *bytes = c->upload_peak;
+  return XT_OK;
+}
 
 int xt_memory_required(xt_ctx* c, long nchol, double* bytes) {
   if(!c || !bytes) return fail(XT_EINVAL, "xt_memory_required: null argument");
@@ -623,6 +634,49 @@ static int check_finite(const char* fn, std::initializer_list<std::pair<const ch
   return XT_OK;
 }
 
+}  // extern "C"
+
+// Counted device allocations of an upload (track_malloc / track_free): the
+// bytes live between the upload's own allocations and frees, and their
+// maximum.  Allocations that rocBLAS makes for itself are not seen.
+namespace {
+struct UploadCount {
+  std::unordered_map<void*, size_t> live;
+  double                            bytes = 0.0, peak = 0.0;
+};
+thread_local UploadCount* g_count = nullptr;
+// Opens a count on this thread for the lifetime of the object.
+struct CountScope {
+  UploadCount c;
+  CountScope() { g_count = &c; }
+  ~CountScope() { g_count = nullptr; }
+};
+}  // namespace
+
+int xt::track_malloc(void** p, size_t bytes) {
+  const hipError_t e = hipMalloc(p, bytes);
+  if(e == hipSuccess && g_count) {
+    g_count->live[*p] = bytes;
+    g_count->bytes += (double) bytes;
+    g_count->peak = std::max(g_count->peak, g_count->bytes);
+  }
+  return (int) e;
+}
+
+void xt::track_free(void* p) {
+  if(!p) return;
+  if(g_count) {
+    auto it = g_count->live.find(p);
+    if(it != g_count->live.end()) {
+      g_count->bytes -= (double) it->second;
+      g_count->live.erase(it);
+    }
+  }
+  (void) hipFree(p);
+}
+
+extern "C" {
+
 static int upload_common(xt_ctx* c, const double* eo, const double* ev, const double* t1,
                          const double* t2, const double* vvoo, const double* ovoo,
                          const double* vvvo) {
@@ -636,14 +690,16 @@ static int upload_common(xt_ctx* c, const double* eo, const double* ev, const do
     if(rc) return rc;
   }
   hipStream_t st = c->stream;
+  CountScope  count;
+  c->upload_peak = 0.0;
   double *    t2_d = nullptr, *ovoo_d = nullptr, *vvvo_d = nullptr;
   auto        cleanup = [&]() {
     for(double* p: {t2_d, ovoo_d, vvvo_d})
-      if(p) (void) hipFree(p);
+      if(p) xt::track_free(p);
     c->free_inputs();
   };
   auto up = [&](double** dst, const double* src, size_t n) -> int {
-    hipError_t e = hipMalloc((void**) dst, n * sizeof(double));
+    hipError_t e = (hipError_t) xt::track_malloc((void**) dst, n * sizeof(double));
     if(e != hipSuccess) {
       *dst = nullptr;
       return fail(XT_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(double)) + " bytes failed");
@@ -665,9 +721,10 @@ static int upload_common(xt_ctx* c, const double* eo, const double* ev, const do
     return fail(err == (int) hipErrorOutOfMemory ? XT_ENOMEM : XT_EHIP,
                 std::string("operand layout: ") + hipGetErrorString((hipError_t) err));
   }
-  (void) hipFree(t2_d);
-  (void) hipFree(ovoo_d);
-  if(vvvo_d) (void) hipFree(vvvo_d);
+  xt::track_free(t2_d);
+  xt::track_free(ovoo_d);
+  if(vvvo_d) xt::track_free(vvvo_d);
+  c->upload_peak = count.c.peak;
   c->have_inputs = true;
   return XT_OK;
 }
@@ -697,14 +754,16 @@ int xt_upload_cholesky(xt_ctx* c, const double* eps_occ, const double* eps_vir, 
     if(rc) return rc;
   }
   hipStream_t st = c->stream;
+  CountScope  count;
+  c->upload_peak = 0.0;
   double *    t2_d = nullptr, *ovoo_d = nullptr, *L_d = nullptr, *none = nullptr;
   auto        cleanup = [&]() {
     for(double* p: {t2_d, ovoo_d, L_d})
-      if(p) (void) hipFree(p);
+      if(p) xt::track_free(p);
     c->free_inputs();
   };
   auto up = [&](double** dst, const double* src, size_t n) -> int {
-    hipError_t e = hipMalloc((void**) dst, n * sizeof(double));
+    hipError_t e = (hipError_t) xt::track_malloc((void**) dst, n * sizeof(double));
     if(e != hipSuccess) {
       *dst = nullptr;
       return fail(XT_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(double)) + " bytes failed");
@@ -724,7 +783,7 @@ int xt_upload_cholesky(xt_ctx* c, const double* eps_occ, const double* eps_vir, 
   }
   int err = xt::cholesky_integrals((int) o, (int) v, nchol, L_d, c->in.vvoo, ovoo_d, c->in.xv, st);
   if(!err) {
-    (void) hipFree(L_d);
+    xt::track_free(L_d);
     L_d = nullptr;
     err = xt::build_operands((int) o, (int) v, t2_d, ovoo_d, none, c->in, st);
   }
@@ -734,8 +793,9 @@ int xt_upload_cholesky(xt_ctx* c, const double* eps_occ, const double* eps_vir, 
     return fail(err == (int) hipErrorOutOfMemory ? XT_ENOMEM : XT_EHIP,
                 std::string("cholesky integrals: ") + hipGetErrorString((hipError_t) err));
   }
-  (void) hipFree(t2_d);
-  (void) hipFree(ovoo_d);
+  xt::track_free(t2_d);
+  xt::track_free(ovoo_d);
+  c->upload_peak = count.c.peak;
   c->have_inputs = true;
   return XT_OK;
 }
@@ -1635,12 +1695,13 @@ int packed_prepare(const char* fn, const PackedCall& P, int* meta, int* n_meta, 
   // K tile lists of the nine bipartitions g = 3*hs + ps: the enabled d1
   // tiles (h7b) with their exec slot for d1 equation 3*((ps+1)%3) + hs, then
   // the enabled d2 tiles (p7b) for d2 equation 3*ps + (hs+1)%3, packed end to
-  // end (ccsd_t_all_fused_cpu.hpp:173-270, 331-428)
+  // end except that the d2 tiles start on a chunk boundary
+  // (ccsd_t_all_fused_cpu.hpp:173-270, 331-428)
   int* counts = m;
   m += xt::kKlistCountInts;
   for(int g = 0; g < 9; g++) {
     const int hs = g / 3, ps = g % 3, s1d = 3 * ((ps + 1) % 3) + hs, s2d = 3 * ps + (hs + 1) % 3;
-    int       n = 0, kb = 0;
+    int       n = 0, nd1 = 0, kb = 0;
     auto      entry = [&](int fam, int ex, int K, int k0) {
       m[0] = fam;
       m[1] = ex;
@@ -1655,15 +1716,17 @@ int packed_prepare(const char* fn, const PackedCall& P, int* meta, int* n_meta, 
       kb += P.d1_size[3 + 7 * tb];
       n++;
     }
+    nd1 = n;
     for(size_t tb = 0; tb < P.nvab; tb++) {
       const int ex = P.d2_exec[s2d + 9 * tb];
       if(ex < 0) continue;
+      if(n == nd1) kb = (kb + xt::kPackedKC - 1) / xt::kPackedKC * xt::kPackedKC;
       entry(1, ex, P.d2_size[6 + 7 * tb], kb);
       kb += P.d2_size[6 + 7 * tb];
       n++;
     }
     entry(0, 0, 0, kb);
-    counts[g] = n;
+    counts[g] = n | nd1 << 16;
   }
   *n_meta = (int) (m - meta);
   for(int i = 0; i < 6; i++) A.dim[i] = (int) P.d[i];

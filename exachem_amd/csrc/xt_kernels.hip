@@ -39,6 +39,7 @@ constexpr int T3N     = 4096;      // 4^6 sub-tile
 constexpr int S1_T1   = 9 * 16;    // s1 T1 blocks in LDS
 constexpr int STAGE_D = 2 * 64 * LDK;  // staging union: A,B chunks; s1 blocks (9*(16+256))
 static_assert(STAGE_D >= S1_T1 + 9 * 256, "staging too small for s1 blocks");
+static_assert(KC == kPackedKC, "the host aligns the per-task d2 tiles to the K chunk");
 
 struct __align__(16) Smem {
   double t3[T3N];
@@ -530,7 +531,7 @@ int build_operands(int O, int V, double* t2, double* ovoo, double*& vvvo, Spatia
                    void* stream) {
   hipStream_t st = (hipStream_t) stream;
   const long  o = O, v = V, op = (O + 7) / 8 * 8, vp = (V + 7) / 8 * 8;
-  auto        alloc = [](double** p, long n) { return hipMalloc((void**) p, sizeof(double) * n); };
+  auto        alloc = [](double** p, long n) { return (hipError_t) track_malloc((void**) p, sizeof(double) * n); };
   hipError_t  e;
   int         rc;
   // d1 operands (contraction over h7, padded to op)
@@ -557,14 +558,14 @@ int build_operands(int O, int V, double* t2, double* ovoo, double*& vvvo, Spatia
   hipLaunchKernelGGL(k_antisym, dim3(nblk(o * o * v * vp)), dim3(256), 0, st, B.t2pa, B.t2p, o, o, v,
                      vp, 0);
   if((e = hipStreamSynchronize(st))) return (int) e;
-  (void) hipFree(tmp);
+  track_free(tmp);
   if(!vvvo) return 0;  // xv comes from elsewhere (build_xv_cholesky), then build_xva
   // d2 M side: xv[c][b][i][a] = vvvo[a][c][b][i], padded in a
   if((e = alloc(&B.xv, v * v * o * vp))) return (int) e;
   if((e = hipMemsetAsync(B.xv, 0, sizeof(double) * v * v * o * vp, st))) return (int) e;
   if((rc = transpose(vvvo, B.xv, v, v * o, v * v * o, vp, v, v * o, v * o * vp, st))) return rc;
   if((e = hipStreamSynchronize(st))) return (int) e;
-  (void) hipFree(vvvo);
+  track_free(vvvo);
   vvvo = nullptr;
   return build_xva(O, V, B, stream);
 }
@@ -573,7 +574,7 @@ int build_xva(int O, int V, SpatialBufs& B, void* stream) {
   hipStream_t st = (hipStream_t) stream;
   const long  o = O, v = V, vp = (V + 7) / 8 * 8;
   hipError_t  e;
-  if((e = hipMalloc((void**) &B.xva, sizeof(double) * v * v * o * vp))) return (int) e;
+  if((e = (hipError_t) track_malloc((void**) &B.xva, sizeof(double) * v * v * o * vp))) return (int) e;
   hipLaunchKernelGGL(k_antisym, dim3(nblk(v * v * o * vp)), dim3(256), 0, st, B.xva, B.xv, v, v, o,
                      vp, 0);
   if((e = hipGetLastError())) return (int) e;
@@ -626,7 +627,7 @@ int launch_reduce(const TaskDev* tasks, int n_tasks, const unsigned long long* a
 __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
   __shared__ Smem sm;
   __shared__ int4 klist[kMaxNoab + kMaxNvab + 1];  // tiles of one bipartition's K, + sentinel
-  const int       tid = threadIdx.x, row = tid >> 2, kq = tid & 3;
+  const int       tid = threadIdx.x;
   const int*      dim = A.dim;
   int             nsub[6], sub[6];
 #pragma unroll
@@ -663,75 +664,108 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
     int       hA, hB, pA, pB;
     pair_of(hs, hA, hB);
     pair_of(ps, pA, pB);
-    const int    s1d = 3 * ((ps + 1) % 3) + hs;  // reference d1 slot of this bipartition
-    const int    s2d = 3 * ps + (hs + 1) % 3;    // reference d2 slot
-    const double sg  = -sgn_single(hs) * sgn_single(ps);
-    const int gHS = 4 * sub[hs] + (row & 3), gPA = 4 * sub[3 + pA] + ((row >> 2) & 3),
-              gPB = 4 * sub[3 + pB] + (row >> 4);
-    const int gHA = 4 * sub[hA] + (row & 3), gHB = 4 * sub[hB] + ((row >> 2) & 3),
-              gPS = 4 * sub[3 + ps] + (row >> 4);
-    const bool mok = gHS < dim[hs] && gPA < dim[3 + pA] && gPB < dim[3 + pB];
-    const bool nok = gHA < dim[hA] && gHB < dim[hB] && gPS < dim[3 + ps];
-    const long DHS = dim[hs], DPA = dim[3 + pA], DPB = dim[3 + pB];
-    const long DHA = dim[hA], DHB = dim[hB], DPS = dim[3 + ps];
-    v4d        acc[2][2];
+    const double sg = -sgn_single(hs) * sgn_single(ps);
+    const long   DHS = dim[hs], DPA = dim[3 + pA], DPB = dim[3 + pB];
+    const long   DHA = dim[hA], DHB = dim[hB], DPS = dim[3 + ps];
+    v4d          acc[2][2];
 #pragma unroll
     for(int i = 0; i < 2; i++)
 #pragma unroll
       for(int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
     // K of this bipartition: the enabled d1 tiles (h7b), then the enabled d2
-    // tiles (p7b), packed end to end and cut into chunks of KC, so that only
-    // the last chunk is padded.  The host lists the tiles (family, exec slot,
-    // K, first packed k) with a sentinel holding the total; each thread walks
-    // the list with its own tile index, which only moves forward since its k
-    // only grow.  Every load is issued (rows and k outside the tile read the
-    // buffer's first element) and masked after it arrives, so the loads of a
-    // chunk are in flight together; the next chunk is loaded into registers
-    // while the current one is multiplied.  Every thread has read the
-    // previous bipartition's list before it is rewritten.
+    // tiles (p7b) from the next multiple of KC, packed end to end and cut into
+    // chunks of KC; each chunk reads one family.  The host lists the tiles
+    // (family, exec slot, K, first packed k) with a sentinel holding the
+    // total.  Every thread has read the previous bipartition's list before it
+    // is rewritten.
     int koff = 0;
-    for(int gg = 0; gg < g; gg++) koff += k_n[gg] + 1;
-    const int nk = k_n[g];
+    for(int gg = 0; gg < g; gg++) koff += (k_n[gg] & 0xffff) + 1;
+    const int nk = k_n[g] & 0xffff, nd1 = k_n[g] >> 16;
     __syncthreads();
     for(int x = tid; x <= nk; x += NT) klist[x] = k_list[koff + x];
     __syncthreads();
-    const int  kall = klist[nk].w;
-    const bool any  = nk > 0;
-    // operand rows of this thread: d1_t2 and d2_v2 / d2_t2 rows are times the
-    // tile's K; the d1_v2 element of k sits at rB1 + k * bstr1
-    const long rA1 = gPA + (gPB + gHS * DPB) * DPA, rB1 = gHB + (gHA + gPS * DHA) * DHB;
+    const int kall = klist[nk].w, kd2 = klist[nd1].w;  // total K, first d2 k
+    // Loads run along k where the packed block keeps k contiguous (d1_t2,
+    // d2_v2, d2_t2: row * K + k): thread (kk, rs) = (tid % 32, tid / 32)
+    // reads k = kk of rows rs + 8 j, so a load instruction reads two runs of
+    // 32 doubles.  The d1 <ij||ka> block keeps k slowest (row + k * bstr1):
+    // there thread (n1, kq) = (tid % 64, tid / 64) reads row n1 at k = 8 kq + j,
+    // and an instruction reads 64 neighbouring rows.
+    // Row r of either operand: r & 3, (r >> 2) & 3, r >> 4 index the first,
+    // second and third t3 position of its side (hs, pA, pB or hA, hB, ps), so
+    // row rs + 8 j moves the second position by 2 (j & 1) and the third by j / 2.
+    const int kk = tid & 31, rs = tid >> 5;
+    const int aHS = 4 * sub[hs] + (rs & 3), aPA = 4 * sub[3 + pA] + (rs >> 2), aPB = 4 * sub[3 + pB];
+    const int bHA = 4 * sub[hA] + (rs & 3), bHB = 4 * sub[hB] + (rs >> 2), bPS = 4 * sub[3 + ps];
+    unsigned  amask = 0, bmask = 0;  // rows inside the block, per j
+#pragma unroll
+    for(int j = 0; j < 8; j++) {
+      if(aHS < DHS && aPA + 2 * (j & 1) < DPA && aPB + (j >> 1) < DPB) amask |= 1u << j;
+      if(bHA < DHA && bHB + 2 * (j & 1) < DHB && bPS + (j >> 1) < DPS) bmask |= 1u << j;
+    }
+    const long a1 = aPA + (aPB + aHS * DPB) * DPA;  // d1_t2 row of j = 0: [hs][pB][pA]
+    const long a2 = aHS + (aPB + aPA * DPB) * DHS;  // d2_v2 row: [pA][pB][hs]
+    const long b2 = bPS + (bHA + bHB * DHA) * DPS;  // d2_t2 row: [hB][hA][ps]
+    const int  n1 = tid & 63, kq = tid >> 6;
+    const int  gHA = 4 * sub[hA] + (n1 & 3), gHB = 4 * sub[hB] + ((n1 >> 2) & 3),
+              gPS = 4 * sub[3 + ps] + (n1 >> 4);
+    const bool nok1  = gHA < DHA && gHB < DHB && gPS < DPS;
+    const long rB1   = gHB + (gHA + gPS * DHA) * DHB;  // d1_v2 element of k = rB1 + k * bstr1
     const long bstr1 = DPS * DHA * DHB;
-    const long rA2 = gHS + (gPB + gPA * DPB) * DHS, rB2 = gPS + (gHA + gHB * DHA) * DPS;
-    int        ti = 0;
-    int4       te = klist[0];
-    int        nbeg = klist[1].w;
-    double     av[8], bv[8];
-    auto       load = [&](int c) {
+    // tile walkers (k only grows): wa for the thread's k-contiguous k, wb for
+    // its d1 <ij||ka> k; te = tile of k, nb = first k of the next tile
+    int  tia = 0, tib = 0;
+    int4 tea = klist[0], teb = klist[0];
+    int  nba = klist[1].w, nbb = klist[1].w;
+    double   av[8], bv[8];
+    unsigned oka = 0, okb = 0;
+    // Issues every load of chunk c (rows outside the block and k outside a
+    // tile read the buffer's first element) and masks after arrival, so the
+    // loads of a chunk are in flight together.
+    auto load = [&](int c) {
+      const bool d2 = c * KC >= kd2;
+      const int  k  = c * KC + kk;
+      bool       kok = k < kall;
+      if(kok)
+        while(k >= nba) {
+          tea = klist[++tia];
+          nba = klist[tia + 1].w;
+        }
+      const long kl = k - tea.w, K = tea.z, ex = tea.y;
+      kok           = kok && kl < K;
       const double* pa[8];
       const double* pb[8];
-      bool          ok[8];
+      {
+        const double* p0 = d2 ? A.d2_v2 + A.max_d2_v2 * ex + a2 * K + kl
+                              : A.d1_t2 + A.max_d1_t2 * ex + a1 * K + kl;
+        const long    s1 = d2 ? 2 * DPB * DHS * K : 2 * K, s2 = d2 ? DHS * K : DPA * K;
+        oka              = kok ? amask : 0u;
 #pragma unroll
-      for(int j = 0; j < 8; j++) {
-        const int k = c * KC + 8 * kq + j;
-        ok[j]       = k < kall;
-        if(ok[j]) {
-          while(k >= nbeg) {
-            te   = klist[++ti];
-            nbeg = klist[ti + 1].w;
-          }
+        for(int j = 0; j < 8; j++) pa[j] = ((oka >> j) & 1) ? p0 + (j & 1) * s1 + (j >> 1) * s2 : A.d1_t2;
+      }
+      if(d2) {
+        const double* p0 = A.d2_t2 + A.max_d2_t2 * ex + b2 * K + kl;
+        const long    s1 = 2 * DHA * DPS * K;
+        okb              = kok ? bmask : 0u;
+#pragma unroll
+        for(int j = 0; j < 8; j++) pb[j] = ((okb >> j) & 1) ? p0 + (j & 1) * s1 + (j >> 1) * K : A.d1_v2;
+      }
+      else {
+        okb = 0;
+#pragma unroll
+        for(int j = 0; j < 8; j++) {
+          const int k1 = c * KC + 8 * kq + j;
+          bool      ok = nok1 && k1 < kall;
+          if(ok)
+            while(k1 >= nbb) {
+              teb = klist[++tib];
+              nbb = klist[tib + 1].w;
+            }
+          const long kl1 = k1 - teb.w;
+          ok             = ok && kl1 < teb.z;
+          pb[j]          = ok ? A.d1_v2 + A.max_d1_v2 * (long) teb.y + rB1 + kl1 * bstr1 : A.d1_v2;
+          okb |= (ok ? 1u : 0u) << j;
         }
-        const long kl = k - te.w, K = te.z;
-        if(te.x == 0) {
-          pa[j] = A.d1_t2 + A.max_d1_t2 * (long) te.y + rA1 * K + kl;
-          pb[j] = A.d1_v2 + A.max_d1_v2 * (long) te.y + rB1 + kl * bstr1;
-        }
-        else {
-          pa[j] = A.d2_v2 + A.max_d2_v2 * (long) te.y + rA2 * K + kl;
-          pb[j] = A.d2_t2 + A.max_d2_t2 * (long) te.y + rB2 * K + kl;
-        }
-        // rows outside the block and k past the end read a valid element
-        pa[j] = (ok[j] && mok) ? pa[j] : A.d1_t2;
-        pb[j] = (ok[j] && nok) ? pb[j] : A.d1_v2;
       }
 #pragma unroll
       for(int j = 0; j < 8; j++) {
@@ -740,27 +774,32 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
       }
 #pragma unroll
       for(int j = 0; j < 8; j++) {
-        av[j] = (ok[j] && mok) ? sg * av[j] : 0.0;
-        bv[j] = (ok[j] && nok) ? bv[j] : 0.0;
+        av[j] = ((oka >> j) & 1) ? sg * av[j] : 0.0;
+        bv[j] = ((okb >> j) & 1) ? bv[j] : 0.0;
       }
     };
-    if(any) {
+    if(kall > 0) {
       const int nc = cdiv(kall, KC);
       load(0);
 #pragma unroll 1
       for(int c = 0; c < nc; c++) {
         __syncthreads();
 #pragma unroll
-        for(int j = 0; j < 8; j++) {
-          As[row * LDK + 8 * kq + j] = av[j];
-          Bs[row * LDK + 8 * kq + j] = bv[j];
+        for(int j = 0; j < 8; j++) As[(rs + 8 * j) * LDK + kk] = av[j];
+        if(c * KC >= kd2) {
+#pragma unroll
+          for(int j = 0; j < 8; j++) Bs[(rs + 8 * j) * LDK + kk] = bv[j];
+        }
+        else {
+#pragma unroll
+          for(int j = 0; j < 8; j++) Bs[n1 * LDK + 8 * kq + j] = bv[j];
         }
         __syncthreads();
         if(c + 1 < nc) load(c + 1);
         mfma_chunk(As, Bs, acc);
       }
+      acc_to_t3(sm, g, acc);
     }
-    if(any) acc_to_t3(sm, g, acc);
   }
 
   // s1 operands from the packed buffers: t1 [hi][pa], v2 [pb][pc][hj][hk]

@@ -232,6 +232,12 @@ class CCSDTriples:
         check(lib().xt_memory_required(self._ctx, nchol, _dp(b)))
         return {"operands": b[0], "staging": b[1], "split": b[2]}
 
+    def upload_peak(self):
+        """Largest device bytes the last upload held at once (its own allocations)."""
+        b = C.c_double()
+        check(lib().xt_upload_peak_bytes(self._ctx, C.byref(b)))
+        return b.value
+
     def upload(self, s):
         arrs = [np.ascontiguousarray(s[k], dtype=np.float64)
                 for k in ("eo", "ev", "t1", "t2", "vvoo", "ovoo", "vvvo")]

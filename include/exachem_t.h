@@ -236,6 +236,14 @@ int xt_run_tasks(xt_ctx* ctx, const xt_opts* opts, long n, const long* which, do
  * otherwise; replaces check_memory_req (hybrid.cpp:19-41). */
 int xt_memory_required(xt_ctx* ctx, long nchol, double* bytes);
 
+/* Largest number of device bytes the last upload (xt_upload_closed_shell,
+ * xt_synth_upload or xt_upload_cholesky) held at once through its own
+ * allocations: resident operands plus the staging still alive (workspace
+ * rocBLAS allocates for itself is not counted); 0 before the first upload.
+ * Host only.  For checking xt_memory_required's operands + staging figure,
+ * which the uploads test against free memory before they allocate. */
+int xt_upload_peak_bytes(xt_ctx* ctx, double* bytes);
+
 /* E[4], E[5] from per-task contributions: task_partials[2*n] in task-list
  * order, factor included, as xt_run returns them.  The sum follows exactly
  * the fixed order of xt_run's device reduction.  A task's contribution is an

@@ -52,6 +52,12 @@ def test_cholesky_v2_matches_host_integrals(xt, O, V, K, occ, vir):
     b = xt.CCSDTriples(0)
     b.set_tiling(tiling)
     b.upload_cholesky(s["eo"], s["ev"], s["t1"], s["t2"], L)
+    # the memory guard's figure covers the Cholesky upload's high point (its
+    # own allocations; rocBLAS workspace aside), as for the regular upload
+    req, peak = b.memory_required(nchol=K), b.upload_peak()
+    print(f"[gpu] cholesky upload peak {peak} B, estimate {req['operands'] + req['staging']} B",
+          flush=True)
+    assert 0 < peak <= req["operands"] + req["staging"] + 2**20
     for prec in (xt.PREC_FP64, xt.PREC_MIXED):
         ra, rb = a.run(precision=prec), b.run(precision=prec)
         print(f"[gpu] cholesky O={O} V={V} K={K} prec {prec}: E4 {ra['e4']!r} / {rb['e4']!r} "

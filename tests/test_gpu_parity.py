@@ -323,9 +323,13 @@ def test_memory_required_matches_allocations(xt):
     ctx.prepare(xt.PREC_MIXED)
     f2, _ = mem_info()
     used_upload, used_split = f0 - f1, f1 - f2
-    print(f"[gpu] measured: upload {used_upload} B, split {used_split} B", flush=True)
+    peak = ctx.upload_peak()
+    print(f"[gpu] measured: upload {used_upload} B, split {used_split} B, upload peak {peak} B "
+          f"(estimate {req['operands'] + req['staging']} B)", flush=True)
     assert abs(used_upload - req["operands"]) <= 0.02 * req["operands"] + 64 * 2**20
     assert abs(used_split - req["split"]) <= 0.02 * req["split"] + 64 * 2**20
+    # the upload's high point (its own allocations) is the guard's operands + staging
+    assert abs(peak - (req["operands"] + req["staging"])) <= 0.02 * peak + 64 * 2**20
     assert req["operands"] + req["staging"] < total
     ctx.close()
 
