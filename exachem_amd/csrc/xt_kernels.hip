@@ -34,7 +34,9 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int NT      = 256;       // threads per workgroup
 constexpr int KC      = 32;        // K chunk staged per LDS round
-constexpr int LDK     = KC + 1;    // padded row stride (doubles) of staged operands
+constexpr int LDK     = KC + 2;    // padded row stride (doubles) of staged operands: 272 B, so
+                                   // the 16x16 fragment reads (16 rows x 2 k per half-wave) hit
+                                   // distinct banks
 constexpr int T3N     = 4096;      // 4^6 sub-tile
 constexpr int S1_T1   = 9 * 16;    // s1 T1 blocks in LDS
 constexpr int STAGE_D = 2 * 64 * LDK;  // staging union: A,B chunks; s1 blocks (9*(16+256))
