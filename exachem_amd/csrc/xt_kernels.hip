@@ -722,8 +722,9 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
     double   av[8], bv[8];
     unsigned oka = 0, okb = 0;
     // Issues every load of chunk c (rows outside the block and k outside a
-    // tile read the buffer's first element) and masks after arrival, so the
-    // loads of a chunk are in flight together.
+    // tile read the buffer's first element; oka / okb mark the valid ones), so
+    // the loads of a chunk are in flight together while the previous chunk is
+    // multiplied.
     auto load = [&](int c) {
       const bool d2 = c * KC >= kd2;
       const int  k  = c * KC + kk;
@@ -774,11 +775,6 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
         av[j] = *pa[j];
         bv[j] = *pb[j];
       }
-#pragma unroll
-      for(int j = 0; j < 8; j++) {
-        av[j] = ((oka >> j) & 1) ? sg * av[j] : 0.0;
-        bv[j] = ((okb >> j) & 1) ? bv[j] : 0.0;
-      }
     };
     if(kall > 0) {
       const int nc = cdiv(kall, KC);
@@ -786,15 +782,17 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
 #pragma unroll 1
       for(int c = 0; c < nc; c++) {
         __syncthreads();
+        // the masks are applied here, so that a chunk's loads are waited for
+        // only after the previous chunk's MFMAs were issued
 #pragma unroll
-        for(int j = 0; j < 8; j++) As[(rs + 8 * j) * LDK + kk] = av[j];
+        for(int j = 0; j < 8; j++) As[(rs + 8 * j) * LDK + kk] = ((oka >> j) & 1) ? sg * av[j] : 0.0;
         if(c * KC >= kd2) {
 #pragma unroll
-          for(int j = 0; j < 8; j++) Bs[(rs + 8 * j) * LDK + kk] = bv[j];
+          for(int j = 0; j < 8; j++) Bs[(rs + 8 * j) * LDK + kk] = ((okb >> j) & 1) ? bv[j] : 0.0;
         }
         else {
 #pragma unroll
-          for(int j = 0; j < 8; j++) Bs[n1 * LDK + 8 * kq + j] = bv[j];
+          for(int j = 0; j < 8; j++) Bs[n1 * LDK + 8 * kq + j] = ((okb >> j) & 1) ? bv[j] : 0.0;
         }
         __syncthreads();
         if(c + 1 < nc) load(c + 1);
