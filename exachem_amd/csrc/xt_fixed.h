@@ -38,6 +38,17 @@ XT_HD inline bool fx_split(double p, unsigned long long& lo, unsigned long long&
   return true;
 }
 
+// Adds the pair (lo, hi) into w[0..1] modulo 2^128, with the carry rule of
+// the kernels' global atomics.  A workgroup sums the partials of its
+// consecutive boxes of one task this way and adds the sum globally once: the
+// same bits as adding each partial globally, since modular addition is exact
+// and associative.
+XT_HD inline void fx_add_local(unsigned long long* w, unsigned long long lo, unsigned long long hi) {
+  const unsigned long long l = w[0] + lo;
+  w[1] += hi + (l < lo ? 1ull : 0ull);
+  w[0] = l;
+}
+
 // Value of an accumulator pair (lo, hi) in Eh, converted through its
 // magnitude so that the result is within a few ulp of the exact sum also for
 // small negative values (hi = -1 with lo near 2^64 would otherwise cancel).

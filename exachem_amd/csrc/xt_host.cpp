@@ -986,6 +986,9 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
                                                                             : nullptr;
   const bool  mixed = o.precision == XT_PREC_MIXED;
   const int*  glog  = mixed ? c->glog_mixed : c->glog_fp64;
+  for(int t: sel)  // the kernels decode a task's tickets in 32-bit arithmetic
+    if(task_tickets(P.dev[t], glog, mixed) >= (1ll << 31))
+      return fail(XT_EINVAL, "run: a task of 2^31 or more box tickets");
   Queues      Q     = build_queues(P, sel, o.schedule, learned, mixed, glog);
   hipStream_t st = c->stream;
   if(!Q.task.empty()) {

@@ -97,18 +97,32 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Adds p into the exact fixed-point pair acc (xt_fixed.h); a partial that is
-// not finite or beyond 2^62 Eh sets *err instead, and the host reports the
-// run as failed.
-__device__ __forceinline__ void fx_add(unsigned long long* acc, double p, unsigned long long* err) {
+// A task's exact fixed-point sums (xt_fixed.h): thread 0 of a workgroup adds
+// its boxes' partials into s[0..3] (E4 lo, hi, E5 lo, hi) and adds them into
+// the task's global pair when it moves on.  A partial that is not finite or
+// beyond 2^62 Eh sets *R.err instead, and the host reports the run as failed.
+__device__ __forceinline__ void sums_add(const RunDev& R, unsigned long long* s, double e4,
+                                         double e5) {
   unsigned long long lo, hi;
-  if(!fx_split(p, lo, hi)) {
-    atomicOr(err, 1ull);
-    return;
-  }
+  if(fx_split(e4, lo, hi)) fx_add_local(&s[0], lo, hi);
+  else atomicOr(R.err, 1ull);
+  if(fx_split(e5, lo, hi)) fx_add_local(&s[2], lo, hi);
+  else atomicOr(R.err, 1ull);
+}
+
+__device__ __forceinline__ void fx_add_global(unsigned long long* acc, unsigned long long lo,
+                                              unsigned long long hi) {
   const unsigned long long old   = atomicAdd(&acc[0], lo);
   const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
   atomicAdd(&acc[1], hi + carry);
+}
+
+__device__ __forceinline__ void sums_flush(const RunDev& R, int task, unsigned long long* s) {
+  if(task >= 0 && (s[0] | s[1] | s[2] | s[3])) {
+    fx_add_global(&R.acc[4 * (size_t) task], s[0], s[1]);
+    fx_add_global(&R.acc[4 * (size_t) task + 2], s[2], s[3]);
+  }
+  s[0] = s[1] = s[2] = s[3] = 0ull;
 }
 
 __device__ __forceinline__ h8 as_h8(uint4 u) { return __builtin_bit_cast(h8, u); }
@@ -618,15 +632,28 @@ __device__ __forceinline__ void box_energy(MSmem& sm, const MixedDev& M, uint64_
   }
 }
 
+// Persistent kernel: workgroup b serves queue b % kQueues first (dispatch is
+// round-robin over the 8 XCDs, so that is its own XCD's queue), then steals
+// from the others; one ticket = one box.  Thread 0 claims and decodes each
+// ticket (its entry searched forward from its previous one) and keeps the
+// running exact sums of the task its boxes belong to, which go to the task's
+// global accumulator when the task changes and when the workgroup leaves.
 __global__ void __launch_bounds__(NT) k_spatial_mixed(MixedDev M, RunDev R) {
-  __shared__ MSmem     sm;
-  __shared__ int       s_task;
-  __shared__ long long s_local;
-  const int            tid  = threadIdx.x;
-  const int            home = blockIdx.x % kQueues;
+  __shared__ MSmem              sm;
+  __shared__ int                s_task;
+  __shared__ int                s_entry;   // thread 0: entry of its last ticket on the current queue
+  __shared__ int                s_cur;     // thread 0: task of the running sums
+  __shared__ unsigned long long s_sum[4];  // thread 0: E4 lo, hi, E5 lo, hi of s_cur not yet added
+  const int                     tid  = threadIdx.x;
+  const int                     home = blockIdx.x % kQueues;
+  if(tid == 0) {
+    s_cur    = -1;
+    s_sum[0] = s_sum[1] = s_sum[2] = s_sum[3] = 0ull;
+  }
 #pragma unroll 1
   for(int qi = 0; qi < kQueues; qi++) {
     const int q = (home + qi) % kQueues;
+    if(tid == 0) s_entry = R.qbeg[q];
 #pragma unroll 1
     for(;;) {
       __syncthreads();
@@ -634,52 +661,60 @@ __global__ void __launch_bounds__(NT) k_spatial_mixed(MixedDev M, RunDev R) {
         const unsigned long long t = atomicAdd(&R.qctr[q], 1ull);
         // progress mirror in host memory (tickets taken so far, approximate)
         if(R.prog) __hip_atomic_store(&R.prog[q], t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        if((long long) t >= R.qtotal[q]) s_task = -1;
-        else {
-          const int e = queue_entry(R.qbeg, R.qtick, q, (long long) t);
-          s_task      = R.qtask[e];
-          s_local     = (long long) t - R.qtick[e];
+        int ti = -1;  // -1: queue exhausted
+        if((long long) t < R.qtotal[q]) {
+          const int e = queue_entry_from(R.qbeg, R.qtick, q, (long long) t, s_entry);
+          s_entry     = e;
+          ti          = R.qtask[e];
+          const TaskDev& T = R.tasks[ti];
+          // ticket -> box: the low bits select the box inside its group
+          int sub[6];
+          if(ticket_box((long long) t - R.qtick[e], T.nsubm, R.glog, sub)) {
+#pragma unroll
+            for(int p = 0; p < 6; p++) {
+              sm.sub[p]  = sub[p];
+              sm.dim[p]  = T.dim[p];
+              sm.off[p]  = T.off[p];
+              sm.tile[p] = T.tile[p];
+            }
+          }
+          else ti = -2;  // a ticket past the task's box grid (its last group overhangs)
         }
+        s_task = ti;
       }
       __syncthreads();
       const int ti = s_task;
-      if(ti < 0) break;
-      const TaskDev& T   = R.tasks[ti];
-      // ticket -> box: the low bits select the box inside its group
-      int sub[6];
-      if(!ticket_box(s_local, T.nsubm, R.glog, sub)) continue;
-#pragma unroll
-      for(int p = 0; p < 6; p++)
-        if(tid == p) {
-          sm.sub[p] = sub[p];
-          sm.dim[p] = T.dim[p];
-          sm.off[p]  = T.off[p];
-          sm.tile[p] = T.tile[p];
-        }
+      if(ti == -1) break;
+      if(ti == -2) continue;
       if(tid >= 64 && tid < 64 + 18) sm.scale[tid - 64] = M.pscale[18 * (size_t) ti + tid - 64];
       {
         float4* t4 = reinterpret_cast<float4*>(sm.t3);
         for(int e = tid; e < BOX / 4; e += NT) t4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
-      __syncthreads();
       if(tid < 36) {
         const int p = tid < 12 ? tid / 4 : 3 + (tid - 12) / 8;
         const int i = tid < 12 ? tid % 4 : (tid - 12) % 8;
         const int g = ext(p) * sm.sub[p] + i;
         sm.eps[tid] = g < sm.dim[p] ? (p < 3 ? M.eo : M.ev)[sm.off[p] + g] : 0.0;
       }
+      __syncthreads();
+      const TaskDev&           T  = R.tasks[ti];
       const unsigned long long t0 = wall_clock64();
       box_contract(sm, M, T.d1code, T.d2code);
       __syncthreads();
       double e4, e5;
       box_energy(sm, M, T.s1code, e4, e5);
       if(tid == 0) {
-        fx_add(&R.acc[4 * (size_t) ti], e4, R.err);
-        fx_add(&R.acc[4 * (size_t) ti + 2], e5, R.err);
+        if(ti != s_cur) {
+          sums_flush(R, s_cur, s_sum);
+          s_cur = ti;
+        }
+        sums_add(R, s_sum, e4, e5);
         if(R.cost) atomicAdd(&R.cost[ti], wall_clock64() - t0);
       }
     }
   }
+  if(tid == 0) sums_flush(R, s_cur, s_sum);
 }
 
 // Per-row exponent: one wave per row, e = 13 - floor(log2(max|x|)); an

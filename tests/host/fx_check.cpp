@@ -1,6 +1,8 @@
 // Host check of the exact fixed-point energy sums (exachem_amd/csrc/xt_fixed.h):
-// split values and add them in several orders with the kernels' carry rule;
-// the totals must be bit-identical across orders, equal to the exactly
+// split values and add them in several orders with the kernels' carry rule,
+// also through running sums over runs of partials (fx_add_local, as a
+// workgroup sums its boxes of one task); the totals must be bit-identical
+// across orders, equal to the exactly
 // rounded sum, and non-finite or out-of-range partials must be refused.
 // Built and run by tests/test_host_safety.py.
 #include <algorithm>
@@ -74,6 +76,23 @@ int main() {
     for(double v: p) {
       xt::fx_split(v, lo, hi);
       add(acc, lo, hi);
+    }
+    CHECK(acc[0] == ref[0] && acc[1] == ref[1]);
+  }
+  // a workgroup's running sum over consecutive partials of one task
+  // (fx_add_local), added once per run of partials, gives the same bits
+  for(int rep = 0; rep < 5; rep++) {
+    std::shuffle(p.begin(), p.end(), g);
+    unsigned long long acc[2] = {0, 0};
+    size_t             i      = 0;
+    while(i < p.size()) {
+      const size_t       run  = 1 + g() % 700;
+      unsigned long long w[2] = {0, 0};
+      for(size_t k = 0; k < run && i < p.size(); k++, i++) {
+        xt::fx_split(p[i], lo, hi);
+        xt::fx_add_local(w, lo, hi);
+      }
+      add(acc, w[0], w[1]);
     }
     CHECK(acc[0] == ref[0] && acc[1] == ref[1]);
   }

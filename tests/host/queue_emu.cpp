@@ -2,8 +2,9 @@
 // (k_spatial_fp64 in xt_kernels.hip, k_spatial_mixed in xt_mixed.hip) over a
 // queue layout the library built (xt_queue_layout).  Workgroup b serves queue
 // b % 8 first, then the others in turn; each claim is one atomic increment
-// of a queue's ticket counter, decoded with the kernels' own queue_entry and
-// ticket_box (exachem_amd/csrc/xt_queue.h).  The order in which workgroups
+// of a queue's ticket counter, decoded with the kernels' own queue_entry_from
+// (checked against queue_entry) and ticket_box (exachem_amd/csrc/xt_queue.h).
+// The order in which workgroups
 // claim is drawn at random (seeded), standing in for any interleaving of the
 // atomics.  Checks: the layout's ticket ranges match task_ticket_count;
 // every box of every selected task is claimed exactly once and no other;
@@ -79,6 +80,8 @@ int main(int argc, char** argv) {
   // the claim loop, one claim per step, workgroups drawn at random
   std::vector<unsigned long long> ctr(xt::kQueues, 0);
   std::vector<int>                qi(grid, 0);  // position in the workgroup's queue order
+  std::vector<int>                hint(grid);   // entry of the workgroup's last ticket on its queue
+  for(int b = 0; b < grid; b++) hint[b] = qbeg[b % xt::kQueues];
   std::vector<long long>          claims(grid, 0), got(grid, 0);
   std::vector<int>                live(grid);
   for(int b = 0; b < grid; b++) live[b] = b;
@@ -95,9 +98,15 @@ int main(int argc, char** argv) {
         live[i] = live.back();
         live.pop_back();
       }
+      else hint[b] = qbeg[(b % xt::kQueues + qi[b]) % xt::kQueues];
       continue;
     }
-    const int       e     = xt::queue_entry(qbeg.data(), qtick.data(), q, (long long) t);
+    // the mixed kernel searches forward from the workgroup's previous entry;
+    // it must find the entry a search over the whole queue finds
+    const int e = xt::queue_entry_from(qbeg.data(), qtick.data(), q, (long long) t, hint[b]);
+    if(e != xt::queue_entry(qbeg.data(), qtick.data(), q, (long long) t))
+      return fail("queue_entry_from", e, (long long) t);
+    hint[b] = e;
     const int       task  = qtask[e];
     const long long local = (long long) t - qtick[e];
     if(e < qbeg[q] || e >= qbeg[q + 1] || local < 0) return fail("queue_entry", e, local);
