@@ -219,7 +219,7 @@ constexpr int kMaxNoab = 50, kMaxNvab = 140;
 // total K in .w).  The first d2 tile starts on a multiple of kPackedKC, the
 // kernel's K chunk, so that every chunk reads one family.
 // kPackedMetaInts is the size at the reference's tile-count limits.
-constexpr int kPackedKC       = 32;
+constexpr int kPackedKC       = 16;
 constexpr int kKlistCountInts = 9;  // 15 + 9 ints after the size/exec arrays: 16-byte aligned
 constexpr int kPackedMetaInts =
   16 * kMaxNoab + 16 * kMaxNvab + 15 + kKlistCountInts + 4 * 9 * (kMaxNoab + kMaxNvab + 1);

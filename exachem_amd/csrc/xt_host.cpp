@@ -1684,6 +1684,11 @@ int packed_prepare(const char* fn, const PackedCall& P, int* meta, int* n_meta, 
   for(size_t b = 0; b < P.nvab; b++)
     if(P.d2_size[7 * b + 6] <= 0 || P.d2_size[7 * b + 6] > 4096)
       return fail(XT_EINVAL, f + ": bad p7 tile size");
+  // the kernel keeps an exec slot in 30 bits of its LDS tile list
+  for(size_t i = 0; i < 9 * P.noab; i++)
+    if(P.d1_exec[i] >= (1 << 30)) return fail(XT_EINVAL, f + ": d1 exec slot out of range");
+  for(size_t i = 0; i < 9 * P.nvab; i++)
+    if(P.d2_exec[i] >= (1 << 30)) return fail(XT_EINVAL, f + ": d2 exec slot out of range");
   int* m   = meta;
   auto put = [&](const int* src, size_t n) {
     std::memcpy(m, src, n * sizeof(int));

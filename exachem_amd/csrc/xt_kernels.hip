@@ -41,7 +41,6 @@ constexpr int T3N     = 4096;      // 4^6 sub-tile
 constexpr int S1_T1   = 9 * 16;    // s1 T1 blocks in LDS
 constexpr int STAGE_D = 2 * 64 * LDK;  // staging union: A,B chunks; s1 blocks (9*(16+256))
 static_assert(STAGE_D >= S1_T1 + 9 * 256, "staging too small for s1 blocks");
-static_assert(KC == kPackedKC, "the host aligns the per-task d2 tiles to the K chunk");
 
 struct __align__(16) Smem {
   double t3[T3N];
@@ -69,7 +68,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Sum of a per-thread value over the workgroup in a fixed order; result valid
 // in thread 0.
-__device__ __forceinline__ void block_sum2(Smem& sm, double& a, double& b) {
+template<class SM>
+__device__ __forceinline__ void block_sum2(SM& sm, double& a, double& b) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   a = wave_sum(a);
   b = wave_sum(b);
@@ -93,18 +93,19 @@ __device__ __forceinline__ int emap(int hs, int hA, int hB, int ps, int pA, int 
 
 // The 64x64xKC MFMA step of one staged chunk: wave w computes rows
 // 32*(w>>1).. and columns 32*(w&1).. as 2x2 16x16 tiles.
+template<int KCT = KC, int LDKT = LDK>
 __device__ __forceinline__ void mfma_chunk(const double* __restrict__ As,
                                            const double* __restrict__ Bs, v4d (&acc)[2][2]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rb = (wave >> 1) * 32, cb = (wave & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for(int kk = 0; kk < KC / 4; kk++) {
+  for(int kk = 0; kk < KCT / 4; kk++) {
     const int k  = 4 * kk + lk;
-    double    a0 = As[(rb + lr) * LDK + k];
-    double    a1 = As[(rb + 16 + lr) * LDK + k];
-    double    b0 = Bs[(cb + lr) * LDK + k];
-    double    b1 = Bs[(cb + 16 + lr) * LDK + k];
+    double    a0 = As[(rb + lr) * LDKT + k];
+    double    a1 = As[(rb + 16 + lr) * LDKT + k];
+    double    b0 = Bs[(cb + lr) * LDKT + k];
+    double    b1 = Bs[(cb + 16 + lr) * LDKT + k];
     acc[0][0]    = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
     acc[0][1]    = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
     acc[1][0]    = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
@@ -115,7 +116,8 @@ __device__ __forceinline__ void mfma_chunk(const double* __restrict__ As,
 // Adds the accumulator of bipartition g into the LDS t3 tile.  D layout of
 // v_mfma_f64_16x16x4f64: register r of lane l holds row 4*r + (l>>4), column
 // l&15 (the four row groups are interleaved, unlike the 16-bit forms).
-__device__ __forceinline__ void acc_to_t3(Smem& sm, int g, const v4d (&acc)[2][2]) {
+template<class SM>
+__device__ __forceinline__ void acc_to_t3(SM& sm, int g, const v4d (&acc)[2][2]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rb = (wave >> 1) * 32, cb = (wave & 1) * 32;
   const int hs = g / 3, ps = g % 3;
@@ -137,10 +139,10 @@ __device__ __forceinline__ void acc_to_t3(Smem& sm, int g, const v4d (&acc)[2][2
 // Energy of the finished sub-tile: t3d in LDS, s1 operands staged in LDS,
 // denominators from sm.eps.  Returns (E4, E5) partial sums in thread 0.
 //   T1s[s][pa][hi] (16 per slot), V1s[s][hk + 4 hj + 16 pc + 64 pb] (256 per slot)
-__device__ __forceinline__ void subtile_energy(Smem& sm, const int* sub, const int* dim,
-                                               unsigned s1mask, double& e4, double& e5) {
-  const double* T1s = sm.stage;
-  const double* V1s = sm.stage + S1_T1;
+template<class SM>
+__device__ __forceinline__ void subtile_energy(SM& sm, const double* T1s, const double* V1s,
+                                               const int* sub, const int* dim, unsigned s1mask,
+                                               double& e4, double& e5) {
   e4 = 0.0;
   e5 = 0.0;
 #pragma unroll 1
@@ -344,7 +346,7 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, uint64_t d1code, 
     V1s[x] = v;
   }
   __syncthreads();
-  subtile_energy(sm, sm.sub, sm.dim, s1mask, e4, e5);
+  subtile_energy(sm, sm.stage, sm.stage + S1_T1, sm.sub, sm.dim, s1mask, e4, e5);
 }
 
 // Adds a double to a 128-bit two's-complement fixed-point accumulator at
@@ -626,12 +628,33 @@ int launch_reduce(const TaskDev* tasks, int n_tasks, const unsigned long long* a
 // (ccsd_t_data_{s1,d1,d2}_new layouts, ccsd_t_all_fused_cpu.hpp index
 // arithmetic), one workgroup per 4^6 block, partials per block.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
-  __shared__ Smem sm;
-  __shared__ int4 klist[kMaxNoab + kMaxNvab + 1];  // tiles of one bipartition's K, + sentinel
-  const int       tid = threadIdx.x;
-  const int*      dim = A.dim;
-  int             nsub[6], sub[6];
+// LDS of the per-task kernel, sized for three workgroups per CU (12 waves):
+// the 4^6 tile, one staged K chunk of 16 (rows padded to 18 doubles, so the
+// 16x16x4 fragment reads of a half-wave hit distinct banks), which later holds
+// the s1 V blocks, and the K tile list of the current bipartition, which
+// later holds the s1 T1 blocks.
+constexpr int PKC  = 16;
+constexpr int PLDK = PKC + 2;
+static_assert(PKC == kPackedKC, "the host aligns the per-task d2 tiles to the K chunk");
+static_assert(2 * 64 * PLDK >= 9 * 256, "staging too small for the s1 V blocks");
+struct __align__(16) PSmem {
+  double t3[T3N];
+  double stage[2 * 64 * PLDK];
+  union {
+    int2   klist[kMaxNoab + kMaxNvab + 1];  // (family | exec slot << 1, K - 1 | first k << 12)
+    double t1s[S1_T1];
+  } u;
+  double eps[24];
+  double red[8];
+};
+// a K tile list entry fits two ints: K <= 4096, total K < 2^20, exec slot < 2^30
+static_assert((kMaxNoab + kMaxNvab) * 4096 + kPackedKC < (1 << 20), "K offsets must fit 20 bits");
+
+__global__ void __launch_bounds__(NT, 3) k_packed_fp64(PackedArgs A) {
+  __shared__ PSmem sm;
+  const int        tid = threadIdx.x;
+  const int*       dim = A.dim;
+  int              nsub[6], sub[6];
 #pragma unroll
   for(int p = 0; p < 6; p++) nsub[p] = cdiv(dim[p], 4);
   {
@@ -658,7 +681,13 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
   }
   __syncthreads();
   double* As = sm.stage;
-  double* Bs = sm.stage + 64 * LDK;
+  double* Bs = sm.stage + 64 * PLDK;
+  // decoded K tile list entry: (family, exec slot, K, first k); first k alone
+  auto kent = [&](int i) {
+    const int2 v = sm.u.klist[i];
+    return make_int4(v.x & 1, v.x >> 1, (v.y & 0xfff) + 1, (int) ((unsigned) v.y >> 12));
+  };
+  auto kbeg = [&](int i) { return (int) ((unsigned) sm.u.klist[i].y >> 12); };
 
 #pragma unroll 1
   for(int g = 0; g < 9; g++) {
@@ -675,35 +704,40 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
 #pragma unroll
       for(int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
     // K of this bipartition: the enabled d1 tiles (h7b), then the enabled d2
-    // tiles (p7b) from the next multiple of KC, packed end to end and cut into
-    // chunks of KC; each chunk reads one family.  The host lists the tiles
-    // (family, exec slot, K, first packed k) with a sentinel holding the
+    // tiles (p7b) from the next multiple of PKC, packed end to end and cut
+    // into chunks of PKC; each chunk reads one family.  The host lists the
+    // tiles (family, exec slot, K, first packed k) with a sentinel holding the
     // total.  Every thread has read the previous bipartition's list before it
     // is rewritten.
     int koff = 0;
     for(int gg = 0; gg < g; gg++) koff += (k_n[gg] & 0xffff) + 1;
     const int nk = k_n[g] & 0xffff, nd1 = k_n[g] >> 16;
     __syncthreads();
-    for(int x = tid; x <= nk; x += NT) klist[x] = k_list[koff + x];
+    for(int x = tid; x <= nk; x += NT) {
+      const int4 e = k_list[koff + x];
+      sm.u.klist[x] = make_int2(e.x | (e.y << 1), (e.z > 0 ? e.z - 1 : 0) | (e.w << 12));
+    }
     __syncthreads();
-    const int kall = klist[nk].w, kd2 = klist[nd1].w;  // total K, first d2 k
+    const int kall = kbeg(nk), kd2 = kbeg(nd1);  // total K, first d2 k
     // Loads run along k where the packed block keeps k contiguous (d1_t2,
-    // d2_v2, d2_t2: row * K + k): thread (kk, rs) = (tid % 32, tid / 32)
-    // reads k = kk of rows rs + 8 j, so a load instruction reads two runs of
-    // 32 doubles.  The d1 <ij||ka> block keeps k slowest (row + k * bstr1):
-    // there thread (n1, kq) = (tid % 64, tid / 64) reads row n1 at k = 8 kq + j,
+    // d2_v2, d2_t2: row * K + k): thread (kk, rs) = (tid % 16, tid / 16)
+    // reads k = kk of rows rs + 16 j, so a load instruction reads four runs of
+    // 16 doubles.  The d1 <ij||ka> block keeps k slowest (row + k * bstr1):
+    // there thread (n1, kq) = (tid % 64, tid / 64) reads row n1 at k = 4 kq + j,
     // and an instruction reads 64 neighbouring rows.
     // Row r of either operand: r & 3, (r >> 2) & 3, r >> 4 index the first,
     // second and third t3 position of its side (hs, pA, pB or hA, hB, ps), so
-    // row rs + 8 j moves the second position by 2 (j & 1) and the third by j / 2.
-    const int kk = tid & 31, rs = tid >> 5;
-    const int aHS = 4 * sub[hs] + (rs & 3), aPA = 4 * sub[3 + pA] + (rs >> 2), aPB = 4 * sub[3 + pB];
-    const int bHA = 4 * sub[hA] + (rs & 3), bHB = 4 * sub[hB] + (rs >> 2), bPS = 4 * sub[3 + ps];
+    // row rs + 16 j moves the third position by j.
+    const int kk = tid & 15, rs = tid >> 4;
+    const int aHS = 4 * sub[hs] + (rs & 3), aPA = 4 * sub[3 + pA] + ((rs >> 2) & 3),
+              aPB = 4 * sub[3 + pB];
+    const int bHA = 4 * sub[hA] + (rs & 3), bHB = 4 * sub[hB] + ((rs >> 2) & 3),
+              bPS = 4 * sub[3 + ps];
     unsigned  amask = 0, bmask = 0;  // rows inside the block, per j
 #pragma unroll
-    for(int j = 0; j < 8; j++) {
-      if(aHS < DHS && aPA + 2 * (j & 1) < DPA && aPB + (j >> 1) < DPB) amask |= 1u << j;
-      if(bHA < DHA && bHB + 2 * (j & 1) < DHB && bPS + (j >> 1) < DPS) bmask |= 1u << j;
+    for(int j = 0; j < 4; j++) {
+      if(aHS < DHS && aPA < DPA && aPB + j < DPB) amask |= 1u << j;
+      if(bHA < DHA && bHB < DHB && bPS + j < DPS) bmask |= 1u << j;
     }
     const long a1 = aPA + (aPB + aHS * DPB) * DPA;  // d1_t2 row of j = 0: [hs][pB][pA]
     const long a2 = aHS + (aPB + aPA * DPB) * DHS;  // d2_v2 row: [pA][pB][hs]
@@ -717,52 +751,51 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
     // tile walkers (k only grows): wa for the thread's k-contiguous k, wb for
     // its d1 <ij||ka> k; te = tile of k, nb = first k of the next tile
     int  tia = 0, tib = 0;
-    int4 tea = klist[0], teb = klist[0];
-    int  nba = klist[1].w, nbb = klist[1].w;
-    double   av[8], bv[8];
+    int4 tea = kent(0), teb = tea;
+    int  nba = kbeg(1), nbb = nba;
+    double   av[4], bv[4];
     unsigned oka = 0, okb = 0;
     // Issues every load of chunk c (rows outside the block and k outside a
     // tile read the buffer's first element; oka / okb mark the valid ones), so
     // the loads of a chunk are in flight together while the previous chunk is
     // multiplied.
     auto load = [&](int c) {
-      const bool d2 = c * KC >= kd2;
-      const int  k  = c * KC + kk;
+      const bool d2 = c * PKC >= kd2;
+      const int  k  = c * PKC + kk;
       bool       kok = k < kall;
       if(kok)
         while(k >= nba) {
-          tea = klist[++tia];
-          nba = klist[tia + 1].w;
+          tea = kent(++tia);
+          nba = kbeg(tia + 1);
         }
       const long kl = k - tea.w, K = tea.z, ex = tea.y;
       kok           = kok && kl < K;
-      const double* pa[8];
-      const double* pb[8];
+      const double* pa[4];
+      const double* pb[4];
       {
         const double* p0 = d2 ? A.d2_v2 + A.max_d2_v2 * ex + a2 * K + kl
                               : A.d1_t2 + A.max_d1_t2 * ex + a1 * K + kl;
-        const long    s1 = d2 ? 2 * DPB * DHS * K : 2 * K, s2 = d2 ? DHS * K : DPA * K;
+        const long    s  = d2 ? DHS * K : DPA * K;
         oka              = kok ? amask : 0u;
 #pragma unroll
-        for(int j = 0; j < 8; j++) pa[j] = ((oka >> j) & 1) ? p0 + (j & 1) * s1 + (j >> 1) * s2 : A.d1_t2;
+        for(int j = 0; j < 4; j++) pa[j] = ((oka >> j) & 1) ? p0 + j * s : A.d1_t2;
       }
       if(d2) {
         const double* p0 = A.d2_t2 + A.max_d2_t2 * ex + b2 * K + kl;
-        const long    s1 = 2 * DHA * DPS * K;
         okb              = kok ? bmask : 0u;
 #pragma unroll
-        for(int j = 0; j < 8; j++) pb[j] = ((okb >> j) & 1) ? p0 + (j & 1) * s1 + (j >> 1) * K : A.d1_v2;
+        for(int j = 0; j < 4; j++) pb[j] = ((okb >> j) & 1) ? p0 + j * K : A.d1_v2;
       }
       else {
         okb = 0;
 #pragma unroll
-        for(int j = 0; j < 8; j++) {
-          const int k1 = c * KC + 8 * kq + j;
+        for(int j = 0; j < 4; j++) {
+          const int k1 = c * PKC + 4 * kq + j;
           bool      ok = nok1 && k1 < kall;
           if(ok)
             while(k1 >= nbb) {
-              teb = klist[++tib];
-              nbb = klist[tib + 1].w;
+              teb = kent(++tib);
+              nbb = kbeg(tib + 1);
             }
           const long kl1 = k1 - teb.w;
           ok             = ok && kl1 < teb.z;
@@ -771,13 +804,13 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
         }
       }
 #pragma unroll
-      for(int j = 0; j < 8; j++) {
+      for(int j = 0; j < 4; j++) {
         av[j] = *pa[j];
         bv[j] = *pb[j];
       }
     };
     if(kall > 0) {
-      const int nc = cdiv(kall, KC);
+      const int nc = cdiv(kall, PKC);
       load(0);
 #pragma unroll 1
       for(int c = 0; c < nc; c++) {
@@ -785,27 +818,27 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
         // the masks are applied here, so that a chunk's loads are waited for
         // only after the previous chunk's MFMAs were issued
 #pragma unroll
-        for(int j = 0; j < 8; j++) As[(rs + 8 * j) * LDK + kk] = ((oka >> j) & 1) ? sg * av[j] : 0.0;
-        if(c * KC >= kd2) {
+        for(int j = 0; j < 4; j++) As[(rs + 16 * j) * PLDK + kk] = ((oka >> j) & 1) ? sg * av[j] : 0.0;
+        if(c * PKC >= kd2) {
 #pragma unroll
-          for(int j = 0; j < 8; j++) Bs[(rs + 8 * j) * LDK + kk] = ((okb >> j) & 1) ? bv[j] : 0.0;
+          for(int j = 0; j < 4; j++) Bs[(rs + 16 * j) * PLDK + kk] = ((okb >> j) & 1) ? bv[j] : 0.0;
         }
         else {
 #pragma unroll
-          for(int j = 0; j < 8; j++) Bs[n1 * LDK + 8 * kq + j] = ((okb >> j) & 1) ? bv[j] : 0.0;
+          for(int j = 0; j < 4; j++) Bs[n1 * PLDK + 4 * kq + j] = ((okb >> j) & 1) ? bv[j] : 0.0;
         }
         __syncthreads();
         if(c + 1 < nc) load(c + 1);
-        mfma_chunk(As, Bs, acc);
+        mfma_chunk<PKC, PLDK>(As, Bs, acc);
       }
       acc_to_t3(sm, g, acc);
     }
   }
 
   // s1 operands from the packed buffers: t1 [hi][pa], v2 [pb][pc][hj][hk]
-  __syncthreads();
-  double*  T1s    = sm.stage;
-  double*  V1s    = sm.stage + S1_T1;
+  __syncthreads();  // every thread is done with the tile list and the staged chunk
+  double*  T1s    = sm.u.t1s;
+  double*  V1s    = sm.stage;
   unsigned s1mask = 0;
   for(int slot = 0; slot < 9; slot++)
     if(s1_exec[slot] >= 0) s1mask |= 1u << slot;
@@ -835,7 +868,7 @@ __global__ void __launch_bounds__(NT) k_packed_fp64(PackedArgs A) {
   }
   __syncthreads();
   double e4, e5;
-  subtile_energy(sm, sub, dim, s1mask, e4, e5);
+  subtile_energy(sm, T1s, V1s, sub, dim, s1mask, e4, e5);
   if(tid == 0) {
     A.partial[blockIdx.x]                = e4;
     A.partial[blockIdx.x + A.num_blocks] = e5;
