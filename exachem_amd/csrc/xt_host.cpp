@@ -1034,7 +1034,9 @@ static int run_units(xt_ctx* c, const xt_opts& o, const std::vector<int>* tasks_
   for(int p = 0; p < 6; p++) R.glog[p] = glog[p];
   long long tickets = 0;
   for(long long t: Q.total) tickets += t;
-  int grid = o.workgroups > 0 ? o.workgroups : c->n_cu * 2;
+  // the FP64 kernel's LDS and registers fit three workgroups on a CU, the
+  // mixed kernel's one
+  int grid = o.workgroups > 0 ? o.workgroups : c->n_cu * 3;
   grid     = (int) std::max<long long>(1, std::min<long long>(grid, tickets));
   xt::MixedDev M = {};
   if(mixed) {

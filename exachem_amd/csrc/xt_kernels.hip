@@ -13,8 +13,9 @@
 //     so the two run back to back into one 64x64 FP64 accumulator held in
 //     MFMA registers (v_mfma_f64_16x16x4f64, 2x2 tiles per wave) and the
 //     result is added once into the LDS t3 tile;
-//   * operands are staged through LDS in K-chunks of 32 (aligned 16-byte
+//   * operands are staged through LDS in K-chunks of 16 (aligned 16-byte
 //     loads from zero-padded arrays) with a register prefetch of the next chunk;
+//     a workgroup holds 53 KB of LDS, so three share a CU;
 //   * s1 (ccsd_t_all_fused_cpu.hpp:472-578) and the denominator-weighted
 //     energy (586-619) run on the LDS tile; the 256 partials are reduced
 //     with wave64 shuffles and a fixed-order cross-wave sum.
@@ -33,18 +34,20 @@ namespace xt {
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int NT      = 256;       // threads per workgroup
-constexpr int KC      = 32;        // K chunk staged per LDS round
-constexpr int LDK     = KC + 2;    // padded row stride (doubles) of staged operands: 272 B, so
+constexpr int KC      = 16;        // K chunk staged per LDS round
+constexpr int LDK     = KC + 2;    // padded row stride (doubles) of staged operands: 144 B, so
                                    // the 16x16 fragment reads (16 rows x 2 k per half-wave) hit
                                    // distinct banks
 constexpr int T3N     = 4096;      // 4^6 sub-tile
 constexpr int S1_T1   = 9 * 16;    // s1 T1 blocks in LDS
-constexpr int STAGE_D = 2 * 64 * LDK;  // staging union: A,B chunks; s1 blocks (9*(16+256))
-static_assert(STAGE_D >= S1_T1 + 9 * 256, "staging too small for s1 blocks");
+constexpr int STAGE_D = 2 * 64 * LDK;  // staging union: A,B chunks; then the s1 V blocks (9*256)
+static_assert(STAGE_D >= 9 * 256, "staging too small for the s1 V blocks");
 
+// LDS of the persistent FP64 kernel, sized for three workgroups per CU.
 struct __align__(16) Smem {
   double t3[T3N];
   double stage[STAGE_D];
+  double t1s[S1_T1];  // s1 T1 blocks
   double eps[24];
   double red[8];
   int    sub[6], dim[6], off[6];  // current sub-tile: position, extents, offsets
@@ -189,16 +192,17 @@ struct RowPtr {
   double        c;
 };
 
-__device__ __forceinline__ void fetch8(const RowPtr& rp, int k0, int Kp, double* x) {
+// Four consecutive k of a row (k0 a multiple of 4; rows are padded to a
+// multiple of 8, so the run stays inside the row).
+__device__ __forceinline__ void fetch4(const RowPtr& rp, int k0, int Kp, double* x) {
   if(rp.c != 0.0 && k0 < Kp) {
-    const double2* q = reinterpret_cast<const double2*>(rp.p + k0);
-    double2        v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3];
+    const double2* q  = reinterpret_cast<const double2*>(rp.p + k0);
+    double2        v0 = q[0], v1 = q[1];
     x[0] = v0.x; x[1] = v0.y; x[2] = v1.x; x[3] = v1.y;
-    x[4] = v2.x; x[5] = v2.y; x[6] = v3.x; x[7] = v3.y;
   }
   else {
 #pragma unroll
-    for(int j = 0; j < 8; j++) x[j] = 0.0;
+    for(int j = 0; j < 4; j++) x[j] = 0.0;
   }
 }
 
@@ -277,16 +281,16 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, uint64_t d1code, 
 #pragma unroll
       for(int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    double    ax[8], bx[8];
+    double    ax[4], bx[4];
     const int nc    = n1 + n2;
     auto      fetch = [&](int c) {
       if(c < n1) {
-        fetch8(a1r, c * KC + 8 * kq, (int) Op, ax);
-        fetch8(b1r, c * KC + 8 * kq, (int) Op, bx);
+        fetch4(a1r, c * KC + 4 * kq, (int) Op, ax);
+        fetch4(b1r, c * KC + 4 * kq, (int) Op, bx);
       }
       else {
-        fetch8(a2r, (c - n1) * KC + 8 * kq, (int) Vp, ax);
-        fetch8(b2r, (c - n1) * KC + 8 * kq, (int) Vp, bx);
+        fetch4(a2r, (c - n1) * KC + 4 * kq, (int) Vp, ax);
+        fetch4(b2r, (c - n1) * KC + 4 * kq, (int) Vp, bx);
       }
     };
     fetch(0);
@@ -295,9 +299,9 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, uint64_t d1code, 
       const double ca = c < n1 ? a1r.c : a2r.c, cb = c < n1 ? b1r.c : b2r.c;
       __syncthreads();
 #pragma unroll
-      for(int j = 0; j < 8; j++) {
-        As[row * LDK + 8 * kq + j] = ca * ax[j];
-        Bs[row * LDK + 8 * kq + j] = cb * bx[j];
+      for(int j = 0; j < 4; j++) {
+        As[row * LDK + 4 * kq + j] = ca * ax[j];
+        Bs[row * LDK + 4 * kq + j] = cb * bx[j];
       }
       __syncthreads();
       if(c + 1 < nc) fetch(c + 1);
@@ -308,8 +312,8 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, uint64_t d1code, 
 
   // ---- s1 operands: T1so(pa,hi) and <hk hj||pc pb> blocks of the sub-tile
   __syncthreads();
-  double*  T1s    = sm.stage;
-  double*  V1s    = sm.stage + S1_T1;
+  double*  T1s    = sm.t1s;
+  double*  V1s    = sm.stage;
   unsigned s1mask = 0;
 #pragma unroll
   for(int slot = 0; slot < 9; slot++)
@@ -346,7 +350,7 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, uint64_t d1code, 
     V1s[x] = v;
   }
   __syncthreads();
-  subtile_energy(sm, sm.stage, sm.stage + S1_T1, sm.sub, sm.dim, s1mask, e4, e5);
+  subtile_energy(sm, T1s, V1s, sm.sub, sm.dim, s1mask, e4, e5);
 }
 
 // Adds a double to a 128-bit two's-complement fixed-point accumulator at
@@ -370,7 +374,7 @@ __device__ __forceinline__ void fx_add(unsigned long long* acc, double p, unsign
 // Persistent kernel: workgroup b serves queue b % kQueues first (dispatch is
 // round-robin over the 8 XCDs, so that is its own XCD's queue), then steals
 // from the others.  One ticket = one sub-tile.
-__global__ void __launch_bounds__(NT) k_spatial_fp64(SpatialDev S, RunDev R) {
+__global__ void __launch_bounds__(NT, 3) k_spatial_fp64(SpatialDev S, RunDev R) {
   __shared__ Smem      sm;
   __shared__ int       s_task;
   __shared__ long long s_local;
