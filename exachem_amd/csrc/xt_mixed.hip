@@ -71,7 +71,7 @@ struct __align__(16) MSmem {
     } d;
     struct {
       float t1[S1P][32];    // [slot][pa*4 + hi]
-      float v1[S1P][1024];  // [slot][hk + 4 hj + 16 pc + 128 pb]
+      float v1[S1P][1088];  // [slot][hk + 4 hj + 16 pc + 128 pb]; p4 slots (hk + 4 hj) * 68 + 8 pc + pb
     } s;
   } u;
   double eps[36];  // h1,h2,h3 (4 each) at 0,4,8; p4,p5,p6 (8 each) at 12,20,28
@@ -458,12 +458,28 @@ __device__ __forceinline__ double s1_sweep(const MSmem& sm) {
     float t[S1P];
 #pragma unroll
     for(int q = 0; q < S1P; q++) t[q] = sm.u.s.t1[q][p4 * 4 + th[q]];
-#pragma unroll 8
-    for(int rr = 0; rr < 64; rr++) {
-      float sv = 0.f;
+    // these slots are stored with pb (= p5) contiguous: the 8 values of one p6
+    // are two 16-byte reads
+#pragma unroll 1
+    for(int p6 = 0; p6 < 8; p6++) {
+      float4 v[S1P][2];
 #pragma unroll
-      for(int q = 0; q < S1P; q++) sv += t[q] * sm.u.s.v1[q][vb[q] + 16 * (rr >> 3) + 128 * (rr & 7)];
-      es += (double) (sm.t3[tid + NT * rr] * sv);
+      for(int q = 0; q < S1P; q++) {
+        const float4* r = reinterpret_cast<const float4*>(&sm.u.s.v1[q][vb[q] * 68 + 8 * p6]);
+        v[q][0]         = r[0];
+        v[q][1]         = r[1];
+      }
+#pragma unroll
+      for(int p5 = 0; p5 < 8; p5++) {
+        float sv = 0.f;
+#pragma unroll
+        for(int q = 0; q < S1P; q++) {
+          const float4 h = v[q][p5 >> 2];
+          const int    i = p5 & 3;
+          sv += t[q] * (i == 0 ? h.x : i == 1 ? h.y : i == 2 ? h.z : h.w);
+        }
+        es += (double) (sm.t3[tid + NT * (p5 + 8 * p6)] * sv);
+      }
     }
   }
   else {
@@ -545,6 +561,10 @@ __device__ __forceinline__ void s1_load(const MSmem& sm, const MixedDev& M, uint
   }
 }
 
+// Stores S1P slots starting at S0.  The slots of particle position p4 (S0 = 0)
+// go to (k + 4 j) * 68 + 8 c + b, so that their sweep reads b (= p5) runs;
+// 68 rather than 64 keeps the stores of a wave on distinct banks but for pairs.
+template<int S0>
 __device__ __forceinline__ void s1_store(MSmem& sm, const S1Regs& R) {
   const int tid = threadIdx.x;
 #pragma unroll
@@ -553,7 +573,9 @@ __device__ __forceinline__ void s1_store(MSmem& sm, const S1Regs& R) {
 #pragma unroll
     for(int j = 0; j < S1V; j++) {
       const int w = tid + NT * j;  // (j, k, c, b) -> stored as k + 4 j + 16 c + 128 b
-      sm.u.s.v1[q][((w >> 2) & 3) + 4 * (w & 3) + (w & ~15)] = (float) R.v[q][j];
+      const int o = S0 == 0 ? (((w >> 2) & 3) + 4 * (w & 3)) * 68 + 8 * ((w >> 4) & 7) + (w >> 7)
+                            : ((w >> 2) & 3) + 4 * (w & 3) + (w & ~15);
+      sm.u.s.v1[q][o] = (float) R.v[q][j];
     }
   }
 }
@@ -597,17 +619,17 @@ __device__ __forceinline__ void box_energy(MSmem& sm, const MixedDev& M, uint64_
   double es = 0.0;
   if(s1code & 0x111111111ull) {
     __syncthreads();  // denominators no longer needed; W complete
-    s1_store(sm, R);
+    s1_store<0>(sm, R);
     s1_load<S1P>(sm, M, s1code, R);
     __syncthreads();
     es += s1_sweep<0>(sm);
     __syncthreads();
-    s1_store(sm, R);
+    s1_store<S1P>(sm, R);
     s1_load<2 * S1P>(sm, M, s1code, R);
     __syncthreads();
     es += s1_sweep<S1P>(sm);
     __syncthreads();
-    s1_store(sm, R);
+    s1_store<2 * S1P>(sm, R);
     __syncthreads();
     es += s1_sweep<2 * S1P>(sm);
   }
