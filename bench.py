@@ -299,7 +299,8 @@ class CpuBaseline(threading.Thread):
     the device runs.  The oracle runs on all host cores but one (the one the
     GPU-driving thread uses).  A task starts only if it is predicted, at the
     rate measured so far, to end before the deadline (seconds from process
-    start); at least one task runs."""
+    start), and only while the timed steps have not ended (`stop`); at least
+    one task runs.  Each finished task prints a line."""
 
     def __init__(self, tiling, s, per_task, deadline_s, n_per_quartile=2):
         super().__init__(daemon=True)
@@ -313,6 +314,7 @@ class CpuBaseline(threading.Thread):
         self.sample = sample[np.argsort(per_task[sample], kind="stable")]
         self.threads = max(1, (os.cpu_count() or 2) - 1)
         self.pick, self.parts, self.secs, self.error = [], [], [], None
+        self.stop = threading.Event()
 
     def run(self):
         try:
@@ -325,13 +327,15 @@ class CpuBaseline(threading.Thread):
             for k in self.sample:
                 if self.pick:
                     rate = float(self.per_task[self.pick].sum()) / sum(self.secs)
-                    if elapsed() + float(self.per_task[k]) / rate > self.deadline:
+                    if self.stop.is_set() or elapsed() + float(self.per_task[k]) / rate > self.deadline:
                         break
                 t0 = time.time()
                 _, _, pt = O.run_spatial(t, evl, self.s, True, [int(k)])
                 self.secs.append(time.time() - t0)
                 self.pick.append(int(k))
                 self.parts.append(pt[0])
+                log(f"cpu baseline: task {int(k)} ({float(self.per_task[k]):.3e} flops) in "
+                    f"{self.secs[-1]:.1f} s")
         except Exception as e:  # reported in the line, never silently
             self.error = repr(e)
 
@@ -608,6 +612,7 @@ def main():
             log("budget reached")
             break
     if cpu is not None:
+        cpu.stop.set()  # no new task after the timed steps; the running one ends
         cpu.join()
         if cpu.error or not cpu.pick:
             raise RuntimeError(f"CPU baseline failed: {cpu.error}")
