@@ -157,6 +157,17 @@ def test_error_codes_not_exits(xt):
     rc = L.xt_fully_fused_ccsd_t_gpu(None, 1, 4, 4, 4, 4, 4, 4, *[None] * 6, *[zp] * 6,
                                      2, 1, 1, 2, 1, 1, 1, 1, *[one] * 6, one, None)
     assert rc == xt.XT_EINVAL
+    # an exec slot the kernel's 30-bit tile-list field cannot hold is refused
+    # before any device work
+    d1s, d2s = np.zeros(7, np.int32), np.zeros(7, np.int32)
+    d1s[3], d2s[6] = 4, 4
+    d1e, d2e, s1e = np.full(9, -1, np.int32), np.full(9, -1, np.int32), np.full(9, -1, np.int32)
+    d1e[0] = 1 << 30
+    s1s = np.full(6, 4, np.int32)
+    arrs = [a.ctypes.data_as(C.POINTER(C.c_int)) for a in (d1s, d1e, d2s, d2e, s1s, s1e)]
+    rc = L.xt_fully_fused_ccsd_t_gpu(None, 1, 4, 4, 4, 4, 4, 4, *[one] * 6, *arrs,
+                                     1, 1, 1, 1, 1, 1, 1, 1, *[one] * 6, one, None)
+    assert rc == xt.XT_EINVAL and b"exec slot" in L.xt_last_error()
 
 
 def test_group_shape_validation(xt):
