@@ -28,6 +28,7 @@
 #include "xt_device.h"
 #include "xt_fixed.h"
 #include "xt_queue.h"
+#include "xt_tasksums.h"
 
 namespace xt {
 
@@ -353,36 +354,27 @@ __device__ void spatial_subtile(Smem& sm, const SpatialDev& S, uint64_t d1code, 
   subtile_energy(sm, T1s, V1s, sm.sub, sm.dim, s1mask, e4, e5);
 }
 
-// Adds a double to a 128-bit two's-complement fixed-point accumulator at
-// scale 2^64 (acc[0] low word, acc[1] high word).  Integer addition is
-// associative, so the final value does not depend on the order of the adds:
-// every task's energy is bit-identical under any schedule.  |p| < 2^53.
-// Adds p into the exact fixed-point pair acc (xt_fixed.h); a partial that is
-// not finite or beyond 2^62 Eh sets *err instead, and the host reports the
-// run as failed.
-__device__ __forceinline__ void fx_add(unsigned long long* acc, double p, unsigned long long* err) {
-  unsigned long long lo, hi;
-  if(!fx_split(p, lo, hi)) {
-    atomicOr(err, 1ull);
-    return;
-  }
-  const unsigned long long old   = atomicAdd(&acc[0], lo);
-  const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
-  atomicAdd(&acc[1], hi + carry);
-}
-
 // Persistent kernel: workgroup b serves queue b % kQueues first (dispatch is
 // round-robin over the 8 XCDs, so that is its own XCD's queue), then steals
-// from the others.  One ticket = one sub-tile.
+// from the others; one ticket = one sub-tile.  Thread 0 claims and decodes each
+// ticket (its entry searched forward from its previous one) and keeps the
+// running exact sums of the task its sub-tiles belong to (xt_tasksums.h).
 __global__ void __launch_bounds__(NT, 3) k_spatial_fp64(SpatialDev S, RunDev R) {
-  __shared__ Smem      sm;
-  __shared__ int       s_task;
-  __shared__ long long s_local;
-  const int            tid  = threadIdx.x;
-  const int            home = blockIdx.x % kQueues;
+  __shared__ Smem               sm;
+  __shared__ int                s_task;
+  __shared__ int                s_entry;   // thread 0: entry of its last ticket on the current queue
+  __shared__ int                s_cur;     // thread 0: task of the running sums
+  __shared__ unsigned long long s_sum[4];  // thread 0: E4 lo, hi, E5 lo, hi of s_cur not yet added
+  const int                     tid  = threadIdx.x;
+  const int                     home = blockIdx.x % kQueues;
+  if(tid == 0) {
+    s_cur    = -1;
+    s_sum[0] = s_sum[1] = s_sum[2] = s_sum[3] = 0ull;
+  }
 #pragma unroll 1
   for(int qi = 0; qi < kQueues; qi++) {
     const int q = (home + qi) % kQueues;
+    if(tid == 0) s_entry = R.qbeg[q];
 #pragma unroll 1
     for(;;) {
       __syncthreads();  // previous sub-tile finished with LDS and s_task
@@ -390,38 +382,45 @@ __global__ void __launch_bounds__(NT, 3) k_spatial_fp64(SpatialDev S, RunDev R) 
         const unsigned long long t = atomicAdd(&R.qctr[q], 1ull);
         // progress mirror in host memory (tickets taken so far, approximate)
         if(R.prog) __hip_atomic_store(&R.prog[q], t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        if((long long) t >= R.qtotal[q]) s_task = -1;
-        else {
-          const int e = queue_entry(R.qbeg, R.qtick, q, (long long) t);
-          s_task      = R.qtask[e];
-          s_local     = (long long) t - R.qtick[e];
+        int ti = -1;  // -1: queue exhausted
+        if((long long) t < R.qtotal[q]) {
+          const int e = queue_entry_from(R.qbeg, R.qtick, q, (long long) t, s_entry);
+          s_entry     = e;
+          ti          = R.qtask[e];
+          const TaskDev& T = R.tasks[ti];
+          // ticket -> sub-tile: the low bits select the box inside its group
+          int sub[6];
+          if(ticket_box((long long) t - R.qtick[e], T.nsub, R.glog, sub)) {
+#pragma unroll
+            for(int p = 0; p < 6; p++) {
+              sm.sub[p] = sub[p];
+              sm.dim[p] = T.dim[p];
+              sm.off[p] = T.off[p];
+            }
+          }
+          else ti = -2;  // a ticket past the task's sub-tile grid (its last group overhangs)
         }
+        s_task = ti;
       }
       __syncthreads();
       const int ti = s_task;
-      if(ti < 0) break;
-      const TaskDev& T = R.tasks[ti];
-      // ticket -> sub-tile: the low bits select the box inside its group
-      int sub[6];
-      if(!ticket_box(s_local, T.nsub, R.glog, sub)) continue;
-#pragma unroll
-      for(int p = 0; p < 6; p++)
-        if(tid == p) {
-          sm.sub[p] = sub[p];
-          sm.dim[p] = T.dim[p];
-          sm.off[p] = T.off[p];
-        }
-      __syncthreads();
+      if(ti == -1) break;
+      if(ti == -2) continue;
+      const TaskDev&           T  = R.tasks[ti];
       const unsigned long long t0 = wall_clock64();
       double                   e4, e5;
       spatial_subtile(sm, S, T.d1code, T.d2code, T.s1code, e4, e5);
       if(tid == 0) {
-        fx_add(&R.acc[4 * (size_t) ti], e4, R.err);
-        fx_add(&R.acc[4 * (size_t) ti + 2], e5, R.err);
+        if(ti != s_cur) {
+          sums_flush(R, s_cur, s_sum);
+          s_cur = ti;
+        }
+        sums_add(R, s_sum, e4, e5);
         if(R.cost) atomicAdd(&R.cost[ti], wall_clock64() - t0);
       }
     }
   }
+  if(tid == 0) sums_flush(R, s_cur, s_sum);
 }
 
 // Per-task energy from the fixed-point accumulators, scaled by the task factor.

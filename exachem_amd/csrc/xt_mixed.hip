@@ -40,6 +40,7 @@
 #include "xt_device.h"
 #include "xt_fixed.h"
 #include "xt_queue.h"
+#include "xt_tasksums.h"
 
 namespace xt {
 namespace {
@@ -95,34 +96,6 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for(int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
-}
-
-// A task's exact fixed-point sums (xt_fixed.h): thread 0 of a workgroup adds
-// its boxes' partials into s[0..3] (E4 lo, hi, E5 lo, hi) and adds them into
-// the task's global pair when it moves on.  A partial that is not finite or
-// beyond 2^62 Eh sets *R.err instead, and the host reports the run as failed.
-__device__ __forceinline__ void sums_add(const RunDev& R, unsigned long long* s, double e4,
-                                         double e5) {
-  unsigned long long lo, hi;
-  if(fx_split(e4, lo, hi)) fx_add_local(&s[0], lo, hi);
-  else atomicOr(R.err, 1ull);
-  if(fx_split(e5, lo, hi)) fx_add_local(&s[2], lo, hi);
-  else atomicOr(R.err, 1ull);
-}
-
-__device__ __forceinline__ void fx_add_global(unsigned long long* acc, unsigned long long lo,
-                                              unsigned long long hi) {
-  const unsigned long long old   = atomicAdd(&acc[0], lo);
-  const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
-  atomicAdd(&acc[1], hi + carry);
-}
-
-__device__ __forceinline__ void sums_flush(const RunDev& R, int task, unsigned long long* s) {
-  if(task >= 0 && (s[0] | s[1] | s[2] | s[3])) {
-    fx_add_global(&R.acc[4 * (size_t) task], s[0], s[1]);
-    fx_add_global(&R.acc[4 * (size_t) task + 2], s[2], s[3]);
-  }
-  s[0] = s[1] = s[2] = s[3] = 0ull;
 }
 
 __device__ __forceinline__ h8 as_h8(uint4 u) { return __builtin_bit_cast(h8, u); }

@@ -101,8 +101,8 @@ int main(int argc, char** argv) {
       else hint[b] = qbeg[(b % xt::kQueues + qi[b]) % xt::kQueues];
       continue;
     }
-    // the mixed kernel searches forward from the workgroup's previous entry;
-    // it must find the entry a search over the whole queue finds
+    // the kernels search forward from the workgroup's previous entry; that
+    // must find the entry a search over the whole queue finds
     const int e = xt::queue_entry_from(qbeg.data(), qtick.data(), q, (long long) t, hint[b]);
     if(e != xt::queue_entry(qbeg.data(), qtick.data(), q, (long long) t))
       return fail("queue_entry_from", e, (long long) t);
