@@ -58,6 +58,9 @@ constexpr int S1P = 3;        // s1 slots staged per sweep
 // the unrolled ring loop picks stage and fragment buffers by s & 1, which is
 // the chunk's parity only when the ring depth is even
 static_assert(PF % 2 == 0, "ring depth must be even");
+// chunk it is multiplied from ring[it % PF] while chunk it+1's scale and
+// chunk it+2's B piece are read from the next two slots
+static_assert(PF >= 3, "the ring must hold the current chunk and the next two");
 
 struct __align__(16) MSmem {
   float t3[BOX];
