@@ -718,7 +718,8 @@ __global__ void __launch_bounds__(NT, 3) k_packed_fp64(PackedArgs A) {
     __syncthreads();
     for(int x = tid; x <= nk; x += NT) {
       const int4 e = k_list[koff + x];
-      sm.u.klist[x] = make_int2(e.x | (e.y << 1), (e.z > 0 ? e.z - 1 : 0) | (e.w << 12));
+      sm.u.klist[x] = make_int2(e.x | (e.y << 1),
+                                (int) ((unsigned) (e.z > 0 ? e.z - 1 : 0) | ((unsigned) e.w << 12)));
     }
     __syncthreads();
     const int kall = kbeg(nk), kd2 = kbeg(nd1);  // total K, first d2 k
