@@ -74,7 +74,8 @@ int xt_device_available(void);
  * callers that wait on it).  partial_energies is device memory of
  * 2*num_blocks doubles, num_blocks = prod(ceil(base_size_x/4)).
  * Limits of the reference (MAX_NOAB=50, MAX_NVAB=140, nontc:56-57) are
- * checked and reported as XT_EINVAL instead of overflowing.
+ * checked and reported as XT_EINVAL instead of overflowing, as are tile
+ * extents above 4096 and exec slot indices of 2^30 or more.
  */
 int xt_fully_fused_ccsd_t_gpu(void* stream, size_t num_blocks, size_t base_size_h1b,
                               size_t base_size_h2b, size_t base_size_h3b, size_t base_size_p4b,
